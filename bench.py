@@ -1,0 +1,251 @@
+"""Throughput of the WaveFD 3D acoustic forward step on one MI355X.
+
+Headline workload: 3D AcoTTIDenQ_DEO2_FDTD forward, FP32, 800^3 at 12.5 m, dt = 1 ms, 50-cell
+sponge, free surface; VTI fields (seeded velocity ramp 1500->4500 m/s +-5% smoothed by a
+15-cell Gaussian, Gardner density, eps in U(0,0.3), eta in U(0,0.15)), tilt in U(-45,45) deg
+and azimuth in U(0,360) deg smoothed by a 25-cell Gaussian, Q varying 30..200 at 10 Hz.
+Velocity is capped so that v*sqrt(1+2 eps)*dt/d <= 0.42 (the 8th-order staggered scheme's
+3D limit is 0.4488).  One "step" is one TimeStep of the full grid, inputs resident in HBM.
+
+Mcells/s = nx*ny*nz*steps / t / 1e6, the reference's definition (benchmark/mcells_per_second.jl:16).
+Roofline: algorithmic HBM bytes of the fused step (TTI: 14 float reads + 4 float writes =
+72 B/cell) / measured kernel time, against MI355X's 8 TB/s.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--physics tti|vti|iso] [--n N]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BYTES_PER_CELL = {"iso": 28, "vti": 56, "tti": 72}
+HBM_PEAK_GBS = 8000.0
+REFERENCE_44T_MCELLS = {"iso": 1961.90, "vti": 893.94, "tti": 800.58}  # docs/src/index.md:42-44
+
+
+def _hip():
+    lib = ctypes.CDLL("libamdhip64.so")
+    lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    lib.hipMemcpy.restype = ctypes.c_int
+    return lib
+
+
+def _copy_d2d(hip, dst, src_tensor):
+    rc = hip.hipMemcpy(dst, src_tensor.data_ptr(), src_tensor.numel() * 4, 3)
+    if rc != 0:
+        raise RuntimeError("hipMemcpy failed: %d" % rc)
+
+
+def _smooth_noise(torch, shape, sigma, seed, dev):
+    """Seeded U(-1,1) noise, separable Gaussian of std sigma (reflecting edges), scaled to max 1."""
+    import torch.nn.functional as F
+    g = torch.Generator(device=dev).manual_seed(seed)
+    x = torch.rand(shape, generator=g, device=dev, dtype=torch.float32) * 2 - 1
+    r = int(3 * sigma)
+    t = torch.arange(-r, r + 1, device=dev, dtype=torch.float32)
+    k = torch.exp(-0.5 * (t / sigma) ** 2)
+    k = (k / k.sum()).view(1, 1, -1)
+    for axis in range(3):
+        xt = x.movedim(axis, -1).contiguous()
+        sh = xt.shape
+        y = F.conv1d(F.pad(xt.view(-1, 1, sh[-1]), (r, r), mode="reflect"), k)
+        x = y.view(sh).movedim(-1, axis)
+        del xt, y
+    x = x.contiguous()
+    return x / x.abs().max()
+
+
+def build_model(torch, physics, n, d, dt, seed, dev):
+    """Return dict of (nx,ny,nz) float32 device tensors for the configured synthetic model."""
+    nx = ny = nz = n
+    shape = (nx, ny, nz)
+    z = torch.arange(nz, device=dev, dtype=torch.float32) * d
+    vz = 1500.0 + 3000.0 * z / ((nz - 1) * d)
+    v = vz.view(1, 1, nz).expand(shape) * (1 + 0.05 * _smooth_noise(torch, shape, 15, seed, dev))
+    m = {}
+    if physics != "iso":
+        m["eps"] = 0.15 + 0.15 * _smooth_noise(torch, shape, 15, seed + 100, dev)
+        m["eta"] = 0.075 + 0.075 * _smooth_noise(torch, shape, 15, seed + 200, dev)
+        m["f"] = torch.full(shape, 0.85, device=dev)
+        vcap = 0.42 * d / (dt * torch.sqrt(1 + 2 * m["eps"]))
+    else:
+        vcap = torch.full(shape, 0.42 * d / dt, device=dev)
+    v = torch.minimum(v, vcap).contiguous()
+    rho = torch.clamp(0.31 * v ** 0.25, 1.0, 2.5)
+    m["v"] = v
+    m["b"] = (1.0 / rho).contiguous()
+    if physics == "tti":
+        th = (np.pi / 4) * _smooth_noise(torch, shape, 25, seed + 300, dev)
+        c = _smooth_noise(torch, shape, 25, seed + 400, dev)
+        s = _smooth_noise(torch, shape, 25, seed + 500, dev)
+        nrm = torch.sqrt(c * c + s * s).clamp_min(1e-6)
+        m["sinTheta"], m["cosTheta"] = torch.sin(th), torch.cos(th)
+        m["sinPhi"], m["cosPhi"] = (s / nrm).contiguous(), (c / nrm).contiguous()
+    return m
+
+
+def run_gpu(physics, n, steps, warmup, seed=3):
+    import torch
+    import wavefd_amd as w
+
+    dev = torch.device("cuda:0")
+    d, dt = 12.5, 1e-3
+    nsponge = 50 if physics == "tti" else 40
+    cls = {"iso": w.Prop3DAcoIsoDenQ_DEO2_FDTD, "vti": w.Prop3DAcoVTIDenQ_DEO2_FDTD,
+           "tti": w.Prop3DAcoTTIDenQ_DEO2_FDTD}[physics]
+    t0 = time.time()
+    prop = cls(nz=n, ny=n, nx=n, dz=d, dy=d, dx=d, dt=dt, nsponge=nsponge, freesurface=True, freqQ=10.0,
+               qMin=0.1, qInterior=100.0)
+    w.coherence(prop, "device")
+    hip = _hip()
+    model = build_model(torch, physics, n, d, dt, seed, dev)
+    for name, t in model.items():
+        _copy_d2d(hip, w.device_ptr(prop, name), t)
+    if physics == "tti":
+        # Q varying 30..200 in the interior; the sponge keeps SetupDtOmegaInvQ's profile
+        q = torch.empty((n, n, n), device=dev)
+        hip.hipMemcpy(q.data_ptr(), w.device_ptr(prop, "dtOmegaInvQ"), q.numel() * 4, 3)
+        qint = 115.0 + 85.0 * _smooth_noise(torch, (n, n, n), 15, seed + 600, dev)
+        k = torch.arange(n, device=dev)
+        dist = torch.minimum(k, n - 1 - k)
+        ks = torch.minimum(torch.minimum(dist.view(-1, 1, 1), dist.view(1, -1, 1)), (n - 1 - k).view(1, 1, -1))
+        q = torch.where(ks >= nsponge, dt * 2 * np.pi * 10.0 / qint, q).contiguous()
+        _copy_d2d(hip, w.device_ptr(prop, "dtOmegaInvQ"), q)
+        del q, qint
+    # a smooth initial wavefield so the arithmetic is not on zeros
+    p0 = _smooth_noise(torch, (n, n, n), 4, seed + 700, dev)
+    for name in ("pCur", "pOld") + (("mCur", "mOld") if physics != "iso" else ()):
+        _copy_d2d(hip, w.device_ptr(prop, name), p0)
+    del model, p0
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+
+    stream = torch.cuda.ExternalStream(prop._lib.GetStream(prop.p))
+    for _ in range(warmup):
+        w.propagateforward(prop)
+    w.synchronize(prop)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        w.propagateforward(prop)
+    ev1.record(stream)
+    w.synchronize(prop)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_events = ev0.elapsed_time(ev1)
+    finite = bool(torch.isfinite(torch.tensor(float(np.float32(0)))))
+    w.free(prop)
+    torch.cuda.empty_cache()
+    cells = float(n) ** 3
+    ms_per_step = wall * 1e3 / steps
+    kernel_ms = ms_events / steps
+    return {
+        "physics": physics, "n": n, "steps": steps, "warmup": warmup, "setup_s": setup_s,
+        "ms_per_step": ms_per_step, "kernel_ms": kernel_ms,
+        "mcells_s": cells * steps / wall / 1e6,
+        "achieved_gbs": cells * BYTES_PER_CELL[physics] / (kernel_ms * 1e-3) / 1e9,
+        "finite": finite,
+    }
+
+
+def cpu_baseline(physics, n, target_s=12.0):
+    """Time the oracle's two-pass step (the reference's structure, nbx=nby=8, nbz=nz) on the
+    host cores for about target_s seconds on an n^3 grid; returns Mcells/s."""
+    from oracle import oracle as orc
+    from tests.helpers import make_fields, make_model
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    cls = {"iso": orc.OracleIso, "vti": orc.OracleVTI, "tti": orc.OracleTTI}[physics]
+    prop = cls(n, n, n, 12.5, 12.5, 12.5, dt=1e-3, fs=True, nsponge=20, freqQ=10.0, qMin=0.1, qInterior=100.0,
+               nthread=threads)
+    shape = (n, n, n)
+    for name, arr in list(make_model(physics, shape, 5).items()) + list(make_fields(physics, shape, 5).items()):
+        getattr(prop, name)[...] = arr
+    prop.time_step()  # warm-up (first touch)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        prop.time_step()
+        k += 1
+        if time.perf_counter() - t0 >= target_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * k / dt / 1e6, "unit": "Mcells/s", "cores": threads, "kind": "port",
+            "sample": "oracle two-pass %s step (reference structure) on %d^3, %d steps in %.1f s" % (
+                physics.upper(), n, k, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--physics", default="tti", choices=["iso", "vti", "tti"])
+    ap.add_argument("--n", type=int, default=800)
+    ap.add_argument("--cpu-n", type=int, default=320)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extra", action="store_true", help="also time Iso 256^3 and VTI 512^3")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the propagator state of one shot lives on one MI355X and this "
+              "build has no multi-GPU path; run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    res = run_gpu(args.physics, args.n, args.steps, args.warmup)
+    extra = []
+    if args.extra:
+        for ph, n in (("iso", 256), ("vti", 512)):
+            r = run_gpu(ph, n, max(10, args.steps), args.warmup)
+            extra.append({k: r[k] for k in ("physics", "n", "mcells_s", "kernel_ms", "achieved_gbs")})
+    cpu = None if args.no_cpu else cpu_baseline(args.physics, args.cpu_n, args.cpu_seconds)
+    frac = res["achieved_gbs"] / HBM_PEAK_GBS
+    line = {
+        "metric": "Mcells/s (3D Iso/TTI DenQ fwd), % HBM roofline; Born shots/h at fixed grad err",
+        "value": round(res["mcells_s"], 2),
+        "unit": "Mcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(res["mcells_s"] / REFERENCE_44T_MCELLS[args.physics], 3),
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D Aco%sDenQ_DEO2_FDTD forward step, %d^3 at 12.5 m, dt 1 ms, free surface, "
+                        "fused single-pass kernel" % (args.physics.upper() if args.physics != "iso" else "Iso", args.n),
+            "cells": args.n ** 3,
+            "vs_baseline_basis": "reference's published 44-thread CPU Mcells/s for this step "
+                                 "(docs/src/index.md:42-44)",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(res["achieved_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(frac, 4),
+            "traffic": None,
+            "bytes_per_cell": BYTES_PER_CELL[args.physics],
+            "kernel_ms": round(res["kernel_ms"], 4),
+        },
+        "cpu_baseline": cpu,
+    }
+    if extra:
+        line["extra"] = extra
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,71 @@
+/*
+ * wavefd_amd.h -- additive entry points of the three MI355X propagator libraries.  None has
+ * a counterpart in the reference; each library exports them with its own prefix
+ * (Prop3DAcoIsoDenQ_DEO2_FDTD_, Prop3DAcoVTIDenQ_DEO2_FDTD_, Prop3DAcoTTIDenQ_DEO2_FDTD_).
+ * They serve callers that keep the time loop on the GPU: device pointers, stream-ordered
+ * calls, registered sources/receivers and one-call step+inject+extract.
+ *
+ * Host coherence.  The reference hands out host arrays (getters) that callers read and
+ * write between calls.  Each handle has a mode:
+ *   1 = COHERENT (default): before a call every host array already handed out by a getter
+ *       and read by the call is copied to the device; after it, every such array the call
+ *       writes is copied back, and the call returns when the work is done.
+ *   0 = DEVICE: no implicit copies and calls are stream-ordered (they may return before the
+ *       work ends); use SyncToDevice / SyncToHost / Synchronize and GetDevicePtr.
+ * Caller-owned arrays passed to Born / imaging calls may be host or device memory in either
+ * mode; host arrays are staged through device scratch.
+ */
+#ifndef WAVEFD_AMD_H
+#define WAVEFD_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct SourcePoint32; /* spacetime.h */
+
+#define WAVEFD_AMD_DECLARE(P)                                                                       \
+    /* set / get the host-coherence mode of a handle (1 coherent, 0 device) */                     \
+    void P##_SetHostCoherence(void *p, long mode);                                                  \
+    long P##_GetHostCoherence(void *p);                                                             \
+    /* copy every host array handed out so far to the device / from the device, and wait */        \
+    void P##_SyncToDevice(void *p);                                                                 \
+    void P##_SyncToHost(void *p);                                                                   \
+    /* wait for all work queued on the handle's stream */                                          \
+    void P##_Synchronize(void *p);                                                                  \
+    /* device address of a state array by name: v b dtOmegaInvQ eps eta f sinTheta cosTheta       \
+     * sinPhi cosPhi pCur pOld mCur mOld pSpace mSpace; NULL if the physics has no such array */  \
+    float *P##_GetDevicePtr(void *p, const char *name);                                             \
+    /* the handle's HIP stream (hipStream_t) */                                                     \
+    void *P##_GetStream(void *p);                                                                   \
+    /* register injection points (as partitions, e.g. from source_blocking) and their data       \
+     * (Julia (nt, nr), host or device); fields: 1 = P, 2 = M, 3 = both.  0 on success. */       \
+    long P##_SetSources(void *p, struct SourcePoint32 **partitions, size_t npartitions, long *npoints, \
+                        float *data, size_t nt, long fields);                                       \
+    /* register extraction points; a zeroed (nt, nr) trace buffer is kept on the device;         \
+     * fields: 1 = P, 2 = M.  0 on success. */                                                     \
+    long P##_SetReceivers(void *p, struct SourcePoint32 **partitions, size_t npartitions, long *npoints, \
+                          size_t nt, long fields);                                                  \
+    /* TimeStep, then inject the sources at it (1-based) into the new pCur (mCur), then extract   \
+     * the receivers at it from the new pCur: the loop body of test/spacetime.jl:486-492 */       \
+    void P##_TimeStepInjectExtract(void *p, long it);                                               \
+    /* copy the receiver traces (Julia (nt, nr)) to out (host or device).  0 on success. */       \
+    long P##_GetReceiverData(void *p, float *out);                                                  \
+    /* 1 (default): fused single-pass step kernel; 0: two-pass operator kernels */                 \
+    void P##_SetFusedStep(void *p, long fused);                                                     \
+    /* last error message of the library ("" if none) and reset */                                 \
+    const char *P##_LastError(void);                                                                \
+    void P##_ClearError(void);
+
+WAVEFD_AMD_DECLARE(Prop3DAcoIsoDenQ_DEO2_FDTD)
+WAVEFD_AMD_DECLARE(Prop3DAcoVTIDenQ_DEO2_FDTD)
+WAVEFD_AMD_DECLARE(Prop3DAcoTTIDenQ_DEO2_FDTD)
+
+#undef WAVEFD_AMD_DECLARE
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -622,39 +622,39 @@ void wo_extract(float *data, const float *field, const wo_source_point32 *pts, l
 /* h is nh filters of 8 taps, row-major h[k*8+l].                             */
 /* ------------------------------------------------------------------------- */
 
-static void interp_fwd_rows(const float *h, float *d, const float *m, long nh, long nd, long nm,
-                            long i, long lo, long hi) {
-    const long j = i * nh;
-    for (long k = 0; k < nh; k++) {
-        const long jk = j + k;
+/* The reference visits coarse rows (0,[3,7]) (1,[2,7]) (2,[1,7]) (3..nd-5,[0,7])
+ * (nd-4,[0,6]) (nd-3,[0,5]) (nd-2,[0,4]) in size_t arithmetic; rows whose index wraps are
+ * skipped by the jk >= nm test and rows repeat when nd < 7.  Kept literally. */
+static void interp_row(int fwd, const float *h, float *d, float *m, size_t nh, size_t nd, size_t nm,
+                       size_t i, size_t lo, size_t hi) {
+    const size_t j = i * nh, im3 = i - 3;
+    for (size_t k = 0; k < nh; k++) {
+        const size_t jk = j + k;
         if (jk >= nm) continue;
-        for (long l = lo; l <= hi; l++) d[i - 3 + l] += h[k * 8 + l] * m[jk];
+        for (size_t l = lo; l <= hi; l++) {
+            if (im3 + l >= nd) continue; /* out of the trace: undefined in the reference (nd < 7) */
+            if (fwd) d[im3 + l] += h[k * 8 + l] * m[jk];
+            else m[jk] += h[k * 8 + l] * d[im3 + l];
+        }
     }
 }
 
-static void interp_adj_rows(const float *h, float *m, const float *d, long nh, long nm, long i,
-                            long lo, long hi) {
-    const long j = i * nh;
-    for (long k = 0; k < nh; k++) {
-        const long jk = j + k;
-        if (jk >= nm) continue;
-        for (long l = lo; l <= hi; l++) m[jk] += h[k * 8 + l] * d[i - 3 + l];
-    }
+static void interp_rows(int fwd, const float *h, float *d, float *m, size_t nh, size_t nd, size_t nm) {
+    interp_row(fwd, h, d, m, nh, nd, nm, 0, 3, 7);
+    interp_row(fwd, h, d, m, nh, nd, nm, 1, 2, 7);
+    interp_row(fwd, h, d, m, nh, nd, nm, 2, 1, 7);
+    for (size_t i = 3; i < nd - 4 && i * nh < nm; i++) interp_row(fwd, h, d, m, nh, nd, nm, i, 0, 7);
+    interp_row(fwd, h, d, m, nh, nd, nm, nd - 4, 0, 6);
+    interp_row(fwd, h, d, m, nh, nd, nm, nd - 3, 0, 5);
+    interp_row(fwd, h, d, m, nh, nd, nm, nd - 2, 0, 4);
 }
 
 /* fine (m, nm samples) -> coarse (d, nd samples), per trace of nslow traces */
 void wo_interp_forward(const float *h, float *d, const float *m, long nh, long nd, long nm, long nslow) {
     for (long s = 0; s < nslow; s++) {
         float *ds = d + s * nd;
-        const float *ms = m + s * nm;
         memset(ds, 0, sizeof(float) * nd);
-        interp_fwd_rows(h, ds, ms, nh, nd, nm, 0, 3, 7);
-        interp_fwd_rows(h, ds, ms, nh, nd, nm, 1, 2, 7);
-        interp_fwd_rows(h, ds, ms, nh, nd, nm, 2, 1, 7);
-        for (long i = 3; i < nd - 4; i++) interp_fwd_rows(h, ds, ms, nh, nd, nm, i, 0, 7);
-        interp_fwd_rows(h, ds, ms, nh, nd, nm, nd - 4, 0, 6);
-        interp_fwd_rows(h, ds, ms, nh, nd, nm, nd - 3, 0, 5);
-        interp_fwd_rows(h, ds, ms, nh, nd, nm, nd - 2, 0, 4);
+        interp_rows(1, h, ds, (float *)(m + s * nm), (size_t)nh, (size_t)nd, (size_t)nm);
     }
 }
 
@@ -662,15 +662,8 @@ void wo_interp_forward(const float *h, float *d, const float *m, long nh, long n
 void wo_interp_adjoint(const float *h, float *m, const float *d, long nh, long nm, long nd, long nslow) {
     for (long s = 0; s < nslow; s++) {
         float *ms = m + s * nm;
-        const float *ds = d + s * nd;
         memset(ms, 0, sizeof(float) * nm);
-        interp_adj_rows(h, ms, ds, nh, nm, 0, 3, 7);
-        interp_adj_rows(h, ms, ds, nh, nm, 1, 2, 7);
-        interp_adj_rows(h, ms, ds, nh, nm, 2, 1, 7);
-        for (long i = 3; i < nd - 4; i++) interp_adj_rows(h, ms, ds, nh, nm, i, 0, 7);
-        interp_adj_rows(h, ms, ds, nh, nm, nd - 4, 0, 6);
-        interp_adj_rows(h, ms, ds, nh, nm, nd - 3, 0, 5);
-        interp_adj_rows(h, ms, ds, nh, nm, nd - 2, 0, 4);
+        interp_rows(0, h, (float *)(d + s * nd), ms, (size_t)nh, (size_t)nd, (size_t)nm);
     }
 }
 

@@ -1,0 +1,299 @@
+"""Spacetime host path (CPU): coefficient/index generation, blocking, and libspacetime's
+host entry points.
+
+Pins: the reference's own spacetime.cc compiled into oracle/_ref (bit-exact, where built),
+the reference tests test/spacetime.jl:87-414 restated (dot products, accuracy, on-grid
+c == 1, partition invariants), and interpolation properties of the Hicks taps.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import wavefd_amd as w
+from wavefd_amd import spacetime as st
+
+F32 = np.float32
+
+
+def _ref_or_skip(oracle):
+    ref = oracle.ref_spacetime()
+    if ref is None:
+        pytest.skip("reference spacetime.cc not built here (needs the reference tree)")
+    return ref
+
+
+# ---- coefficients ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("F", ["hicks", "linear"])
+def test_on_grid_points_give_one_unit_tap(F):
+    """test/spacetime.jl:323-339"""
+    rng = np.random.default_rng(0)
+    dz = dy = dx = 10.0
+    nz, ny, nx = 100, 101, 102
+    x = rng.integers(5, nx - 5, 25) * dx
+    y = rng.integers(5, ny - 5, 25) * dy
+    z = rng.integers(5, nz - 5, 25) * dz
+    fn = w.hickscoeffs if F == "hicks" else w.linearcoeffs
+    pts = fn(F32(dz), F32(dy), F32(dx), 0.0, 0.0, 0.0, nz, ny, nx, z, y, x)
+    blks = w.source_blocking(pts, 48)
+    assert sum(len(b) for b in blks) == 25
+    for b in blks:
+        assert np.all(b["c"] == 1)
+    # iu is Julia's LinearIndices((nz,ny,nx))[iz,iy,ix]
+    iu = (z / dz + 1) + (y / dy) * nz + (x / dx) * nz * ny
+    assert sorted(np.concatenate(blks)["iu"].tolist()) == sorted(iu.astype(np.int64).tolist())
+
+
+def _taps_interp(pts, f, dz, dy, dx, nz, ny):
+    iu = pts["iu"] - 1
+    iz = iu % nz
+    iy = (iu // nz) % ny
+    ix = iu // (nz * ny)
+    return float(np.sum(pts["c"].astype(np.float64) * f(iz * dz, iy * dy, ix * dx)))
+
+
+def test_hicks_taps_interpolate_smooth_fields():
+    """Kaiser-windowed sinc taps reproduce a band-limited field at the off-grid point
+    (Hicks 2002); a wrong sign or offset of alpha breaks this."""
+    rng = np.random.default_rng(1)
+    dz, dy, dx = 10.0, 12.0, 8.0
+    nz, ny, nx = 60, 60, 60
+
+    def f(z, y, x):
+        return np.cos(2 * np.pi * z / 97.0 + 0.3) * np.cos(2 * np.pi * y / 131.0) * np.sin(2 * np.pi * x / 83.0 + 1)
+
+    for _ in range(10):
+        zs, ys, xs = rng.uniform(150, 400), rng.uniform(150, 400), rng.uniform(150, 400)
+        pts = w.hickscoeffs(F32(dz), F32(dy), F32(dx), 0.0, 0.0, 0.0, nz, ny, nx, [zs], [ys], [xs])
+        assert len(pts) <= 512
+        got = _taps_interp(pts, f, dz, dy, dx, nz, ny)
+        assert abs(got - f(zs, ys, xs)) < 2e-3
+
+
+def test_hicks_free_surface_image_makes_taps_odd():
+    """Near the free surface the image terms (src/spacetime.jl:905-917) make the taps
+    interpolate the odd extension: fields vanishing at the surface are reproduced."""
+    dz = dy = dx = 10.0
+    nz, ny, nx = 40, 40, 40
+
+    def f(z, y, x):  # odd about z = 0 (the surface row iz = 1)
+        return np.sin(2 * np.pi * z / 90.0) * np.cos(2 * np.pi * y / 120.0) * np.cos(2 * np.pi * x / 110.0)
+
+    for zs in (3.0, 7.5, 14.0, 21.0, 26.0):
+        pts = w.hickscoeffs(F32(dz), F32(dy), F32(dx), 0.0, 0.0, 0.0, nz, ny, nx, [zs], [200.0], [200.0])
+        assert pts["iu"].min() >= 1
+        got = _taps_interp(pts, f, dz, dy, dx, nz, ny)
+        assert abs(got - f(zs, 200.0, 200.0)) < 3e-3, zs
+
+
+def test_linear_taps_are_trilinear_and_zero_on_the_surface():
+    dz = dy = dx = 10.0
+    pts = w.linearcoeffs(F32(dz), F32(dy), F32(dx), 0.0, 0.0, 0.0, 30, 30, 30, [123.0], [47.0], [281.0])
+    assert abs(float(pts["c"].astype(np.float64).sum()) - 1) < 1e-6
+    lin = lambda z, y, x: 0.3 * z - 0.2 * y + 0.1 * x + 2.0  # noqa: E731
+    assert abs(_taps_interp(pts, lin, dz, dy, dx, 30, 30) - lin(123.0, 47.0, 281.0)) < 1e-4
+    # a source between the surface row (iz=1) and iz=2: the surface taps get no weight
+    pts = w.linearcoeffs(F32(dz), F32(dy), F32(dx), 0.0, 0.0, 0.0, 30, 30, 30, [4.0], [47.0], [281.0])
+    iz = (pts["iu"] - 1) % 30
+    assert np.all(iz == 1)
+
+
+# ---- blocking --------------------------------------------------------------------------------
+
+def test_blocking_invariants():
+    """test/spacetime.jl:199-300: at most nthreads partitions, no key split across partitions."""
+    rng = np.random.default_rng(2)
+    z, y, x = rng.uniform(50, 900, 100), rng.uniform(50, 900, 100), rng.uniform(50, 900, 100)
+    pts = w.hickscoeffs(F32(10), F32(10), F32(10), 0.0, 0.0, 0.0, 100, 100, 100, z, y, x)
+    for nthreads in (1, 3, 48):
+        for key, fn in (("iu", w.source_blocking), ("ir", w.receiver_blocking)):
+            blks = fn(pts, nthreads)
+            assert len(blks) <= nthreads
+            assert sum(len(b) for b in blks) == len(pts)
+            seen = {}
+            for i, b in enumerate(blks):
+                for k in set(b[key].tolist()):
+                    assert seen.setdefault(k, i) == i
+
+
+def test_repeated_points_make_expected_blocks():
+    """test/spacetime.jl:360-388: 3 identical on-grid points, 4 threads -> 3 points, all in
+    one partition, the partition count never above the thread count."""
+    pts = w.hickscoeffs(F32(10), F32(10), F32(10), 0.0, 0.0, 0.0, 50, 50, 50, [100.0] * 3, [100.0] * 3, [100.0] * 3)
+    assert len(pts) == 3
+    blks = w.source_blocking(pts, 4)
+    assert len(blks) == 1 and len(blks[0]) == 3
+
+
+# ---- libspacetime host path vs the reference build -------------------------------------------
+
+def _rand_points(rng, n, nz, ny, nx, nr):
+    pts = np.zeros(n, st.SourcePoint32)
+    pts["iu"] = rng.integers(1, nz * ny * nx + 1, n)
+    pts["ir"] = rng.integers(1, nr + 1, n)
+    pts["c"] = rng.standard_normal(n).astype(F32)
+    return pts
+
+
+def _call_ref(ref, name, *args):
+    fn = getattr(ref, name)
+    fn.restype = None
+    fn(*args)
+
+
+def test_inject_extract_bit_exact_vs_reference(oracle):
+    ref = _ref_or_skip(oracle)
+    rng = np.random.default_rng(3)
+    nz, ny, nx, nr, nt = 20, 21, 22, 7, 50
+    pts = _rand_points(rng, 300, nz, ny, nx, nr)
+    # repeated keys stay together and in order after blocking
+    for it in (1, 17, 50):
+        blks = w.source_blocking(pts, 4)
+        data = rng.standard_normal((nr, nt)).astype(F32)
+        field = rng.standard_normal(nz * ny * nx).astype(F32)
+        a = field.copy()
+        w.injectdata(a, blks, data, it)
+        b = field.copy()
+        keep, pp, np_, npts = st._partition_arrays(blks)
+        _call_ref(ref, "injectdata_float", ctypes.c_void_p(b.ctypes.data), ctypes.c_void_p(data.ctypes.data),
+                  ctypes.c_size_t(it), ctypes.c_size_t(nt), ctypes.c_void_p(pp), ctypes.c_size_t(np_),
+                  ctypes.c_void_p(npts), ctypes.c_size_t(1))
+        assert np.array_equal(a, b)
+        # the oracle restatement agrees too
+        c = field.copy()
+        oracle.inject(c, data, it, np.concatenate(blks))
+        assert np.array_equal(a, c)
+
+        rblks = w.receiver_blocking(pts, 4)
+        d1 = rng.standard_normal((nr, nt)).astype(F32)
+        d2 = d1.copy()
+        d3 = d1.copy()
+        w.extractdata(d1, field, it, rblks)
+        keep, pp, np_, npts = st._partition_arrays(rblks)
+        _call_ref(ref, "extractdata_float", ctypes.c_void_p(d2.ctypes.data), ctypes.c_void_p(field.ctypes.data),
+                  ctypes.c_void_p(pp), ctypes.c_size_t(np_), ctypes.c_void_p(npts), ctypes.c_size_t(it),
+                  ctypes.c_size_t(nt), ctypes.c_size_t(1))
+        assert np.array_equal(d1, d2)
+        oracle.extract(d3, field, it, np.concatenate(rblks))
+        assert np.array_equal(d1, d3)
+
+
+@pytest.mark.parametrize("F", ["hicks", "linear"])
+def test_inject_extract_dot_product(F):
+    """test/spacetime.jl:390-414 in 3D: <f1, E g1> = <I f1, g1>."""
+    rng = np.random.default_rng(0)
+    nr, nz, ny, nx = 10, 40, 41, 42
+    z = rng.uniform(50, 340, nr)
+    y = rng.uniform(50, 350, nr)
+    x = rng.uniform(50, 360, nr)
+    fn = w.hickscoeffs if F == "hicks" else w.linearcoeffs
+    pts = fn(F32(10), F32(10), F32(10), 0.0, 0.0, 0.0, nz, ny, nx, z, y, x)
+    f1 = rng.random((nr, 1)).astype(F32)
+    f2 = np.zeros((nr, 1), F32)
+    g1 = rng.random(nz * ny * nx).astype(F32)
+    g2 = np.zeros(nz * ny * nx, F32)
+    w.injectdata(g2, w.source_blocking(pts, 4), f1, 1)
+    w.extractdata(f2, g1, 1, w.receiver_blocking(pts, 4))
+    lhs = float(np.dot(f1.ravel().astype(np.float64), f2.ravel()))
+    rhs = float(np.dot(g1.astype(np.float64), g2))
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs + rhs)
+
+
+# ---- time interpolation ----------------------------------------------------------------------
+
+def test_interpfilters_unit_rows_for_equal_rates():
+    h = w.interpfilters(0.004, 0.004)
+    assert h.shape == (1, 8)
+    assert abs(h[0, 3] - 1) < 1e-6 and np.abs(np.delete(h[0], 3)).max() < 1e-6
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_interp_dot_product(mode):
+    """test/spacetime.jl:87-101"""
+    rng = np.random.default_rng(4)
+    h = w.interpfilters(0.001, 0.004, mode)
+    m = rng.random((5, 256)).astype(F32)
+    d = rng.random((5, 64)).astype(F32)
+    ds = np.zeros_like(d)
+    ms = np.zeros_like(m)
+    w.interpforward(h, ds, m)
+    w.interpadjoint(h, ms, d)
+    lhs = float(np.dot(m.ravel().astype(np.float64), ms.ravel()))
+    rhs = float(np.dot(ds.ravel().astype(np.float64), d.ravel()))
+    assert abs(lhs - rhs) <= np.finfo(F32).eps * m.size
+
+
+def test_interp_accuracy():
+    """test/spacetime.jl:131-197"""
+    td = 0.004 * np.arange(64)
+    tm = 0.001 * np.arange(256)
+    dd = (np.sin(10 * td) + 0.2 * np.sin(30 * td)).astype(F32)[None, :]
+    mm = (np.sin(10 * tm) + 0.2 * np.sin(30 * tm)).astype(F32)[None, :]
+    m = np.zeros_like(mm)
+    w.interpadjoint(w.interpfilters(0.001, 0.004, 0), m, dd)
+    m *= 4
+    assert np.linalg.norm((m - mm)[:, 8:256 - 8]) / m.size < 0.1
+    # away from the trimmed end rows the 8-tap sinc is accurate to ~1e-3
+    assert np.abs((m - mm)[:, 32:256 - 32]).max() < 2e-3
+    d = np.zeros_like(dd)
+    w.interpforward(w.interpfilters(0.001, 0.004, 0), d, mm)
+    assert np.linalg.norm((d - dd)[:, 8:64 - 8]) / d.size < 0.1
+    assert np.abs((d - dd)[:, 8:64 - 8]).max() < 2e-3
+
+
+# the reference indexes past the coarse trace for nd < 7 (undefined behaviour): compare nd >= 7
+@pytest.mark.parametrize("nd", [64, 9, 8, 7])
+def test_interp_bit_exact_vs_reference(oracle, nd):
+    ref = _ref_or_skip(oracle)
+    rng = np.random.default_rng(nd)
+    h = w.interpfilters(0.001, 0.004, 0)
+    nm = 4 * nd - 2
+    m = rng.standard_normal((3, nm)).astype(F32)
+    d = rng.standard_normal((3, nd)).astype(F32)
+    dout = np.zeros_like(d)
+    w.interpforward(h, dout, m)
+    rows, hp = st._hptrs(h)
+    dref = np.zeros_like(d)
+    _call_ref(ref, "interpforward_nd_float", ctypes.c_void_p(ctypes.addressof(hp)), ctypes.c_void_p(dref.ctypes.data),
+              ctypes.c_void_p(m.ctypes.data), ctypes.c_size_t(4), ctypes.c_size_t(nd), ctypes.c_size_t(nm),
+              ctypes.c_size_t(3), ctypes.c_size_t(1))
+    assert np.array_equal(dout, dref)
+    dor = np.zeros_like(d)
+    oracle.lib().wo_interp_forward(oracle._p(h), oracle._p(dor), oracle._p(m), 4, nd, nm, 3)
+    assert np.array_equal(dor, dref)
+    mout = np.zeros_like(m)
+    w.interpadjoint(h, mout, d)
+    mref = np.zeros_like(m)
+    _call_ref(ref, "interpadjoint_nd_float", ctypes.c_void_p(ctypes.addressof(hp)), ctypes.c_void_p(mref.ctypes.data),
+              ctypes.c_void_p(d.ctypes.data), ctypes.c_size_t(4), ctypes.c_size_t(nm), ctypes.c_size_t(nd),
+              ctypes.c_size_t(3), ctypes.c_size_t(1))
+    assert np.array_equal(mout, mref)
+    assert np.array_equal(oracle.interp_adjoint(h, d, nm), mref)
+
+
+@pytest.mark.parametrize("nd", [2, 3, 4, 5, 6])
+def test_interp_short_traces_are_memory_safe(nd):
+    """Short coarse traces (undefined in the reference) run without touching memory outside
+    the traces, and forward/adjoint stay transposes of each other."""
+    rng = np.random.default_rng(nd)
+    h = w.interpfilters(0.001, 0.004, 0)
+    m = rng.standard_normal((2, 4 * nd)).astype(F32)
+    d = rng.standard_normal((2, nd)).astype(F32)
+    ds = np.zeros_like(d)
+    ms = np.zeros_like(m)
+    w.interpforward(h, ds, m)
+    w.interpadjoint(h, ms, d)
+    lhs = float(np.dot(m.ravel().astype(np.float64), ms.ravel()))
+    rhs = float(np.dot(ds.ravel().astype(np.float64), d.ravel()))
+    assert abs(lhs - rhs) <= 1e-5 * (abs(lhs) + abs(rhs) + 1)
+
+
+def test_ricker_closed_form():
+    t = np.linspace(-0.1, 0.1, 201)
+    r = w.ricker(15.0, t)
+    assert r[100] == 1.0
+    assert abs(t[np.argmin(r)]) > 0 and np.all(np.abs(r) <= 1.0 + 1e-7)
+    # zeros at t = +-1/(sqrt(2) pi f)
+    t0 = 1 / (np.sqrt(2) * np.pi * 15.0)
+    assert abs(float(w.ricker(15.0, np.array([t0]))[0])) < 1e-6

@@ -1,0 +1,151 @@
+"""ctypes loading of the in-tree HIP libraries with the reference's C signatures.
+
+The product path has no CPU fallback: if a library is missing, loading raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+
+_L, _F, _S = ctypes.c_long, ctypes.c_float, ctypes.c_size_t
+_P = ctypes.c_void_p
+_FP = ctypes.c_void_p  # float* passed as an integer address (host or device)
+
+PREFIXES = {
+    "iso": "Prop3DAcoIsoDenQ_DEO2_FDTD",
+    "vti": "Prop3DAcoVTIDenQ_DEO2_FDTD",
+    "tti": "Prop3DAcoTTIDenQ_DEO2_FDTD",
+}
+
+# name -> (restype, argtypes); common to the three propagator libraries
+_COMMON = {
+    "alloc": (_P, [_L, _L, _L, _L, _L, _L, _F, _F, _F, _F, _L, _L, _L]),
+    "free": (None, [_P]),
+    "SetupDtOmegaInvQ": (None, [_P, _F, _F, _F]),
+    "TimeStep": (None, [_P]),
+    "ScaleSpatialDerivatives": (None, [_P]),
+    "getNx": (_L, [_P]),
+    "getNy": (_L, [_P]),
+    "getNz": (_L, [_P]),
+    # additive device path
+    "SetHostCoherence": (None, [_P, _L]),
+    "GetHostCoherence": (_L, [_P]),
+    "SyncToDevice": (None, [_P]),
+    "SyncToHost": (None, [_P]),
+    "Synchronize": (None, [_P]),
+    "GetDevicePtr": (_P, [_P, ctypes.c_char_p]),
+    "GetStream": (_P, [_P]),
+    "SetSources": (_L, [_P, _P, _S, _P, _P, _S, _L]),
+    "SetReceivers": (_L, [_P, _P, _S, _P, _S, _L]),
+    "TimeStepInjectExtract": (None, [_P, _L]),
+    "GetReceiverData": (_L, [_P, _P]),
+    "SetFusedStep": (None, [_P, _L]),
+    "LastError": (ctypes.c_char_p, []),
+    "ClearError": (None, []),
+}
+
+_PHYS = {
+    "iso": {
+        "ForwardBornInjection_V": (None, [_P, _FP, _FP]),
+        "ForwardBornInjection_VB": (None, [_P, _FP, _FP, _FP, _FP]),
+        "ForwardBornInjection_B": (None, [_P, _FP, _FP, _FP]),
+        "AdjointBornAccumulation_V": (None, [_P, _FP, _FP]),
+        "AdjointBornAccumulation_VB": (None, [_P, _FP, _FP, _FP, _FP]),
+        "AdjointBornAccumulation_B": (None, [_P, _FP, _FP, _FP]),
+        "AdjointBornAccumulation_wavefieldsep": (None, [_P, _FP, _FP, _L]),
+        "getV": (_P, [_P]), "getB": (_P, [_P]), "getDtOmegaInvQ": (_P, [_P]),
+        "getPSpace": (_P, [_P]), "getPCur": (_P, [_P]), "getPOld": (_P, [_P]),
+    },
+    "vti": {
+        "TimeStepLinear": (None, [_P]),
+        "ForwardBornInjection_V": (None, [_P, _FP, _FP, _FP]),
+        "ForwardBornInjection_VEA": (None, [_P, _FP, _FP, _FP, _FP, _FP, _FP, _FP]),
+        "AdjointBornAccumulation_V": (None, [_P, _FP, _FP, _FP]),
+        "AdjointBornAccumulation_wavefieldsep_V": (None, [_P, _FP, _FP, _FP, _L]),
+        "AdjointBornAccumulation_VEA": (None, [_P, _FP, _FP, _FP, _FP, _FP, _FP, _FP]),
+        "getV": (_P, [_P]), "getEps": (_P, [_P]), "getEta": (_P, [_P]), "getB": (_P, [_P]),
+        "getF": (_P, [_P]), "getPSpace": (_P, [_P]), "getMSpace": (_P, [_P]), "getPCur": (_P, [_P]),
+        "getPOld": (_P, [_P]), "getMCur": (_P, [_P]), "getMOld": (_P, [_P]), "getDtOmegaInvQ": (_P, [_P]),
+    },
+}
+_PHYS["tti"] = {k: v for k, v in _PHYS["vti"].items() if k != "TimeStepLinear"}
+for _a in ("SinTheta", "CosTheta", "SinPhi", "CosPhi"):
+    _PHYS["tti"]["get" + _a] = (_P, [_P])
+
+SPACETIME = {
+    "injectdata_float": (None, [_P, _P, _S, _S, _P, _S, _P, _S]),
+    "injectdata_double": (None, [_P, _P, _S, _S, _P, _S, _P, _S]),
+    "extractdata_float": (None, [_P, _P, _P, _S, _P, _S, _S, _S]),
+    "extractdata_double": (None, [_P, _P, _P, _S, _P, _S, _S, _S]),
+    "interpadjoint_1d_float": (None, [_P, _P, _P, _S, _S, _S]),
+    "interpadjoint_1d_double": (None, [_P, _P, _P, _S, _S, _S]),
+    "interpforward_1d_float": (None, [_P, _P, _P, _S, _S, _S]),
+    "interpforward_1d_double": (None, [_P, _P, _P, _S, _S, _S]),
+    "interpadjoint_nd_float": (None, [_P, _P, _P, _S, _S, _S, _S, _S]),
+    "interpadjoint_nd_double": (None, [_P, _P, _P, _S, _S, _S, _S, _S]),
+    "interpforward_nd_float": (None, [_P, _P, _P, _S, _S, _S, _S, _S]),
+    "interpforward_nd_double": (None, [_P, _P, _P, _S, _S, _S, _S, _S]),
+    "spacetime_LastError": (ctypes.c_char_p, []),
+}
+
+
+def signatures(kind):
+    """{symbol: (restype, argtypes)} for 'iso' | 'vti' | 'tti' | 'spacetime'."""
+    if kind == "spacetime":
+        return dict(SPACETIME)
+    pre = PREFIXES[kind]
+    out = {}
+    for name, sig in list(_COMMON.items()) + list(_PHYS[kind].items()):
+        out[pre + "_" + name] = sig
+    return out
+
+
+def lib_path(kind):
+    if kind == "spacetime":
+        name = "libspacetime.so"
+    else:  # the reference's library names start in lower case: libprop3DAco...
+        pre = PREFIXES[kind]
+        name = "lib" + pre[0].lower() + pre[1:] + ".so"
+    return os.path.join(LIBDIR, name)
+
+
+_cache = {}
+
+
+class _Bound:
+    """Attribute access to a library's symbols without the common prefix."""
+
+    def __init__(self, kind, dll):
+        self.kind = kind
+        self.dll = dll
+        self.prefix = "" if kind == "spacetime" else PREFIXES[kind] + "_"
+
+    def __getattr__(self, name):
+        fn = getattr(self.dll, self.prefix + name)
+        setattr(self, name, fn)
+        return fn
+
+
+def load(kind):
+    """Load (once) and type one of the HIP libraries; raises when it has not been built."""
+    if kind in _cache:
+        return _cache[kind]
+    path = lib_path(kind)
+    if not os.path.exists(path):
+        raise RuntimeError("%s is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+                           "or `python wavefd_amd/build.py`" % path)
+    dll = ctypes.CDLL(path)
+    for sym, (res, args) in signatures(kind).items():
+        fn = getattr(dll, sym)
+        fn.restype = res
+        fn.argtypes = args
+    b = _Bound(kind, dll)
+    _cache[kind] = b
+    return b
+
+
+def last_error(kind):
+    lib = load(kind)
+    msg = lib.spacetime_LastError() if kind == "spacetime" else lib.LastError()
+    return msg.decode() if msg else ""

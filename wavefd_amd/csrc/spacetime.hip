@@ -1,0 +1,290 @@
+// libspacetime -- drop-in for the reference's src/spacetime.cc (built by src/CMakeLists.txt:13).
+//
+// Same entry points, argument order and C types (declared with citations in
+// include/spacetime.h).  Each entry point dispatches on where the caller's arrays live:
+//   * host arrays (the reference's case: Julia arrays, or the host mirrors the propagator
+//     getters return) are processed on the host, with the reference's loop structure and
+//     accumulation order, because the data is there (one thread: the result does not depend
+//     on the reference's nthreads, which only partitions race-free work);
+//   * device arrays (GetDevicePtr, or any hipMalloc'd buffer) are processed by gfx950
+//     kernels on the null stream, ordered after prior work on that stream and completed
+//     before the call returns.
+// Injection / extraction accumulate per key in the reference's partition order: all points
+// sharing a grid index (injection) or a trace (extraction) sit in one partition in order
+// (src/spacetime.jl:1087-1129), and the device kernels give one thread per key group.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct SourcePoint32 {
+    long iu;
+    long ir;
+    float c;
+};
+struct SourcePoint64 {
+    long iu;
+    long ir;
+    double c;
+};
+
+std::string g_err;
+
+void report(const std::string &s) {
+    g_err = s;
+    std::fprintf(stderr, "[wavefd_amd spacetime] %s\n", s.c_str());
+}
+
+bool is_device(const void *ptr) {
+    if (!ptr) return false;
+    hipPointerAttribute_t a;
+    std::memset(&a, 0, sizeof(a));
+    if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+template <class P, class T>
+__global__ void k_inject(T *__restrict__ field, const T *__restrict__ data, long it, long nt, const P *__restrict__ pts,
+                         const int *__restrict__ gs, int ng) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ng) return;
+    const long iu = pts[gs[gi]].iu - 1;
+    T acc = field[iu];
+    for (int i = gs[gi]; i < gs[gi + 1]; i++) acc += pts[i].c * data[(pts[i].ir - 1) * nt + it - 1];
+    field[iu] = acc;
+}
+
+template <class P, class T>
+__global__ void k_extract(T *__restrict__ data, const T *__restrict__ field, long it, long nt,
+                          const P *__restrict__ pts, const int *__restrict__ gs, int ng) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ng) return;
+    const long id = (pts[gs[gi]].ir - 1) * nt + it - 1;
+    T acc = data[id];
+    for (int i = gs[gi]; i < gs[gi + 1]; i++) acc += pts[i].c * field[pts[i].iu - 1];
+    data[id] = acc;
+}
+
+// Group the partitions' points by key (stable, so per-key order is the partition order),
+// upload, launch, wait.  Used when the field is device memory.
+template <class P, class T>
+void device_scatter_gather(bool inject, T *field, T *data, size_t it, size_t nt, P **parts, size_t nparts,
+                           const long *npts) {
+    // the device field's allocation bounds the grid index: reject points that would land outside
+    hipDeviceptr_t base = nullptr;
+    size_t span = 0;
+    if (hipMemGetAddressRange(&base, &span, (hipDeviceptr_t)field) != hipSuccess) {
+        (void)hipGetLastError();
+        report("cannot determine the size of the device field");
+        return;
+    }
+    const long ncells = (long)((span - (size_t)((char *)field - (char *)base)) / sizeof(T));
+    std::vector<P> flat;
+    for (size_t i = 0; i < nparts; i++)
+        for (long j = 0; j < npts[i]; j++) {
+            const P &q = parts[i][j];
+            if (q.iu < 1 || q.iu > ncells || q.ir < 1) {
+                report("point outside the device field (iu=" + std::to_string(q.iu) + ")");
+                return;
+            }
+            flat.push_back(q);
+        }
+    if (flat.empty()) return;
+    std::stable_sort(flat.begin(), flat.end(),
+                     [inject](const P &a, const P &b) { return inject ? a.iu < b.iu : a.ir < b.ir; });
+    std::vector<int> gs;
+    long maxir = 0;
+    for (size_t i = 0; i < flat.size(); i++) {
+        const long key = inject ? flat[i].iu : flat[i].ir;
+        if (i == 0 || key != (inject ? flat[i - 1].iu : flat[i - 1].ir)) gs.push_back((int)i);
+        maxir = std::max(maxir, flat[i].ir);
+    }
+    gs.push_back((int)flat.size());
+    P *dpts = nullptr;
+    int *dgs = nullptr;
+    T *ddata = nullptr;
+    const bool data_dev = is_device(data);
+    const size_t dbytes = sizeof(T) * (size_t)maxir * nt;
+    hipError_t e = hipMalloc(&dpts, sizeof(P) * flat.size());
+    if (e == hipSuccess) e = hipMalloc(&dgs, sizeof(int) * gs.size());
+    if (e == hipSuccess) e = hipMemcpy(dpts, flat.data(), sizeof(P) * flat.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dgs, gs.data(), sizeof(int) * gs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !data_dev) {
+        e = hipMalloc(&ddata, dbytes);
+        if (e == hipSuccess) e = hipMemcpy(ddata, data, dbytes, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) {
+        T *dd = data_dev ? data : ddata;
+        const int ng = (int)gs.size() - 1;
+        if (inject)
+            k_inject<P, T><<<(ng + 255) / 256, 256>>>(field, dd, (long)it, (long)nt, dpts, dgs, ng);
+        else
+            k_extract<P, T><<<(ng + 255) / 256, 256>>>(dd, field, (long)it, (long)nt, dpts, dgs, ng);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess && !inject && !data_dev) e = hipMemcpy(data, ddata, dbytes, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) report(std::string(inject ? "injectdata" : "extractdata") + ": " + hipGetErrorString(e));
+    if (dpts) hipFree(dpts);
+    if (dgs) hipFree(dgs);
+    if (ddata) hipFree(ddata);
+}
+
+// ---- host paths: src/spacetime.cc:144-204 ---------------------------------------------
+
+template <class P, class T>
+void host_inject(T *field, const T *data, size_t it, size_t nt, P **parts, size_t nparts, const long *npts,
+                 size_t nthreads) {
+    for (size_t ip = 0; ip < nparts; ip++)
+        for (long j = 0; j < npts[ip]; j++) {
+            const P p = parts[ip][j];
+            field[p.iu - 1] += p.c * data[(p.ir - 1) * nt + it - 1];
+        }
+}
+
+template <class P, class T>
+void host_extract(T *data, const T *field, P **parts, size_t nparts, const long *npts, size_t it, size_t nt,
+                  size_t nthreads) {
+    for (size_t ip = 0; ip < nparts; ip++)
+        for (long j = 0; j < npts[ip]; j++) {
+            const P p = parts[ip][j];
+            data[(p.ir - 1) * nt + it - 1] += p.c * field[p.iu - 1];
+        }
+}
+
+// ---- 8-tap polyphase time interpolation: src/spacetime.cc:4-127 -------------------------
+// The reference visits coarse rows in a fixed sequence with trimmed tap ranges at both
+// ends: (0,[3,7]) (1,[2,7]) (2,[1,7]) (3..nd-5,[0,7]) (nd-4,[0,6]) (nd-3,[0,5]) (nd-2,[0,4]),
+// in size_t arithmetic.  The sequence is kept literally -- including rows visited twice
+// when nd < 7 and rows whose index wraps, which the jk >= nm test then skips.  For nd < 7
+// the reference also reads/writes d past its end (undefined behaviour); here those taps are
+// skipped, which leaves every in-range result unchanged.
+
+struct Row {
+    size_t i, lo, hi;
+};
+
+inline std::vector<Row> interp_rows(size_t nd, size_t nh, size_t nm) {
+    std::vector<Row> rows = {{0, 3, 7}, {1, 2, 7}, {2, 1, 7}};
+    // rows with i*nh >= nm touch nothing, so stopping there changes no result (and avoids
+    // the reference's near-endless loop when nd < 4 makes nd-4 wrap)
+    for (size_t i = 3; i < nd - 4 && i * nh < nm; i++) rows.push_back({i, 0, 7});
+    rows.push_back({nd - 4, 0, 6});
+    rows.push_back({nd - 3, 0, 5});
+    rows.push_back({nd - 2, 0, 4});
+    return rows;
+}
+
+template <class T>
+void host_interp_forward_1d(T **h, T *d, const T *m, size_t nh, size_t nd, size_t nm) {
+    std::memset(d, 0, nd * sizeof(T));
+    for (const Row &row : interp_rows(nd, nh, nm)) {
+        const size_t j = row.i * nh, im3 = row.i - 3;
+        for (size_t k = 0; k < nh; k++) {
+            const size_t jk = j + k;
+            if (jk >= nm) continue;
+            for (size_t l = row.lo; l <= row.hi; l++)
+                if (im3 + l < nd) d[im3 + l] += h[k][l] * m[jk];
+        }
+    }
+}
+
+template <class T>
+void host_interp_adjoint_1d(T **h, T *m, const T *d, size_t nh, size_t nm, size_t nd) {
+    std::memset(m, 0, nm * sizeof(T));
+    for (const Row &row : interp_rows(nd, nh, nm)) {
+        const size_t j = row.i * nh, im3 = row.i - 3;
+        for (size_t k = 0; k < nh; k++) {
+            const size_t jk = j + k;
+            if (jk >= nm) continue;
+            for (size_t l = row.lo; l <= row.hi; l++)
+                if (im3 + l < nd) m[jk] += h[k][l] * d[im3 + l];
+        }
+    }
+}
+
+}  // namespace
+
+#pragma GCC visibility push(default)
+extern "C" {
+
+void injectdata_float(float *field, float *data, size_t it, size_t nt, SourcePoint32 **partitions,
+                      size_t npartitions, long *npoints_per_partition, size_t nthreads) {
+    if (is_device(field))
+        device_scatter_gather<SourcePoint32, float>(true, field, data, it, nt, partitions, npartitions,
+                                                    npoints_per_partition);
+    else
+        host_inject(field, data, it, nt, partitions, npartitions, npoints_per_partition, nthreads);
+}
+
+void injectdata_double(double *field, double *data, size_t it, size_t nt, SourcePoint64 **partitions,
+                       size_t npartitions, long *npoints_per_partition, size_t nthreads) {
+    if (is_device(field))
+        device_scatter_gather<SourcePoint64, double>(true, field, data, it, nt, partitions, npartitions,
+                                                     npoints_per_partition);
+    else
+        host_inject(field, data, it, nt, partitions, npartitions, npoints_per_partition, nthreads);
+}
+
+void extractdata_float(float *data, float *field, SourcePoint32 **partitions, size_t npartitions,
+                       long *npoints_per_partition, size_t it, size_t nt, size_t nthreads) {
+    if (is_device(field))
+        device_scatter_gather<SourcePoint32, float>(false, field, data, it, nt, partitions, npartitions,
+                                                    npoints_per_partition);
+    else
+        host_extract(data, field, partitions, npartitions, npoints_per_partition, it, nt, nthreads);
+}
+
+void extractdata_double(double *data, double *field, SourcePoint64 **partitions, size_t npartitions,
+                        long *npoints_per_partition, size_t it, size_t nt, size_t nthreads) {
+    if (is_device(field))
+        device_scatter_gather<SourcePoint64, double>(false, field, data, it, nt, partitions, npartitions,
+                                                     npoints_per_partition);
+    else
+        host_extract(data, field, partitions, npartitions, npoints_per_partition, it, nt, nthreads);
+}
+
+void interpadjoint_1d_float(float **h, float *m, float *d, size_t nh, size_t nm, size_t nd) {
+    host_interp_adjoint_1d(h, m, d, nh, nm, nd);
+}
+void interpadjoint_1d_double(double **h, double *m, double *d, size_t nh, size_t nm, size_t nd) {
+    host_interp_adjoint_1d(h, m, d, nh, nm, nd);
+}
+void interpforward_1d_float(float **h, float *d, float *m, size_t nh, size_t nd, size_t nm) {
+    host_interp_forward_1d(h, d, m, nh, nd, nm);
+}
+void interpforward_1d_double(double **h, double *d, double *m, size_t nh, size_t nd, size_t nm) {
+    host_interp_forward_1d(h, d, m, nh, nd, nm);
+}
+
+void interpadjoint_nd_float(float **h, float *m, float *d, size_t nh, size_t nm, size_t nd, size_t nslow,
+                            size_t nthreads) {
+    for (size_t i = 0; i < nslow; i++) host_interp_adjoint_1d(h, m + i * nm, d + i * nd, nh, nm, nd);
+}
+void interpadjoint_nd_double(double **h, double *m, double *d, size_t nh, size_t nm, size_t nd, size_t nslow,
+                             size_t nthreads) {
+    for (size_t i = 0; i < nslow; i++) host_interp_adjoint_1d(h, m + i * nm, d + i * nd, nh, nm, nd);
+}
+void interpforward_nd_float(float **h, float *d, float *m, size_t nh, size_t nd, size_t nm, size_t nslow,
+                            size_t nthreads) {
+    std::memset(d, 0, nd * nslow * sizeof(float));
+    for (size_t i = 0; i < nslow; i++) host_interp_forward_1d(h, d + i * nd, m + i * nm, nh, nd, nm);
+}
+void interpforward_nd_double(double **h, double *d, double *m, size_t nh, size_t nd, size_t nm, size_t nslow,
+                             size_t nthreads) {
+    std::memset(d, 0, nd * nslow * sizeof(double));
+    for (size_t i = 0; i < nslow; i++) host_interp_forward_1d(h, d + i * nd, m + i * nm, nh, nd, nm);
+}
+
+const char *spacetime_LastError(void) { return g_err.c_str(); }
+
+}  // extern "C"
+#pragma GCC visibility pop

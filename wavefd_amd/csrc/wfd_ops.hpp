@@ -1,0 +1,369 @@
+// wfd_ops.hpp -- the operations behind each C-ABI entry point, written once for the three
+// physics.  Every function brackets its device work with Prop::begin/end so the
+// reference's host-visible semantics hold in COHERENT mode (see wfd_runtime.hpp).
+#pragma once
+#include <hipfft/hipfft.h>
+
+#include "wfd_fused.hpp"
+#include "wfd_runtime.hpp"
+
+namespace wfd {
+
+// ---------------------------------------------------------------------------------------
+// time step
+// ---------------------------------------------------------------------------------------
+
+// Two-pass step exactly as the reference structures it: sandwich into temporaries, then
+// D- + update (src/prop3DAcoIsoDenQ_DEO2_FDTD.h:247-261, src/prop3DAcoVTIDenQ_DEO2_FDTD.h:326-376,
+// src/prop3DAcoTTIDenQ_DEO2_FDTD.h:359-379).  Kept as the cross-check of the fused kernels.
+static inline bool step_two_pass(Prop *p, bool linear) {
+    const Grid &g = p->g;
+    const float dt2 = p->dt * p->dt;
+    float *t[6];
+    const int nt = p->phys == ISO ? 3 : 6;
+    for (int i = 0; i < nt; i++)
+        if (!(t[i] = p->scratch(i))) return false;
+    Field *f = p->f;
+    if (p->phys == ISO) {
+        k_iso_sandwich<<<op_grid(g), op_block(), 0, p->stream>>>(g, f[F_PCUR].d, f[F_B].d, t[0], t[1], t[2]);
+        WFD_LAUNCHED();
+        k_iso_update<<<op_grid(g), op_block(), 0, p->stream>>>(g, dt2, t[0], t[1], t[2], f[F_V].d, f[F_B].d,
+                                                               f[F_Q].d, f[F_PCUR].d,
+                                                               linear ? nullptr : f[F_PSPACE].d, f[F_POLD].d);
+        WFD_LAUNCHED();
+        return true;
+    }
+    if (p->phys == VTI) {
+        k_vti_sandwich<<<op_grid(g), op_block(), 0, p->stream>>>(g, f[F_PCUR].d, f[F_MCUR].d, f[F_EPS].d, f[F_ETA].d,
+                                                                 f[F_F].d, f[F_B].d, t[0], t[1], t[2], t[3], t[4],
+                                                                 t[5]);
+    } else {
+        k_tti_sandwich<<<op_grid(g), op_block(), 0, p->stream>>>(
+            g, f[F_PCUR].d, f[F_MCUR].d, f[F_ST].d, f[F_CT].d, f[F_SP].d, f[F_CP].d, f[F_EPS].d, f[F_ETA].d, f[F_F].d,
+            f[F_B].d, t[0], t[1], t[2], t[3], t[4], t[5]);
+    }
+    WFD_LAUNCHED();
+    k_vti_update<<<op_grid(g), op_block(), 0, p->stream>>>(
+        g, dt2, t[0], t[1], t[2], t[3], t[4], t[5], f[F_V].d, f[F_B].d, f[F_Q].d, f[F_PCUR].d, f[F_MCUR].d,
+        linear ? nullptr : f[F_PSPACE].d, linear ? nullptr : f[F_MSPACE].d, f[F_POLD].d, f[F_MOLD].d);
+    WFD_LAUNCHED();
+    return true;
+}
+
+static inline bool step_device(Prop *p, bool linear) {
+    bool ok = p->fused ? step_fused(p, linear) : step_two_pass(p, linear);
+    if (!ok) return false;
+    p->swap_p();
+    return true;
+}
+
+static inline Mask step_writes(Prop *p, bool linear) {
+    // after the swap the new field is pCur; the previous pCur became pOld (unchanged data)
+    Mask m = bit(F_PCUR);
+    if (!linear) m |= bit(F_PSPACE);
+    if (p->phys != ISO) {
+        m |= bit(F_MCUR);
+        if (!linear) m |= bit(F_MSPACE);
+    }
+    return m;
+}
+
+static inline bool time_step(Prop *p, bool linear) {
+    if (!p->begin(p->all_mask())) return false;
+    if (!step_device(p, linear)) return false;
+    return p->end(step_writes(p, linear));
+}
+
+// step, then inject the registered sources at time index it into the new pCur (and mCur),
+// then extract the registered receivers at it -- the loop of test/spacetime.jl:486-492
+// plus extraction, as one call.
+static inline bool time_step_inject_extract(Prop *p, long it, bool linear) {
+    if (!p->begin(p->all_mask())) return false;
+    if (!step_device(p, linear)) return false;
+    if (!inject_registered(p, it)) return false;
+    if (!extract_registered(p, it)) return false;
+    return p->end(step_writes(p, linear));
+}
+
+// ---------------------------------------------------------------------------------------
+// Born source injection
+// ---------------------------------------------------------------------------------------
+
+#define WFD_ARG(ptr, slot, in, out, a) \
+    if (!p->arg((ptr), (slot), (in), (out), (a))) return false
+
+static inline bool iso_inj_v(Prop *p, float *dV, float *DP) {
+    if (!p->begin(bit(F_V) | bit(F_B) | bit(F_PCUR))) return false;
+    Prop::Arg a[2];
+    WFD_ARG(dV, 0, true, false, a[0]);
+    WFD_ARG(DP, 1, true, false, a[1]);
+    k_iso_inj_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, p->f[F_V].d, p->f[F_B].d, a[0].dev, a[1].dev,
+                                                           p->f[F_PCUR].d);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, 2)) return false;
+    return p->end(bit(F_PCUR));
+}
+
+// VB (dV != null) and B (dV == null) sources (src/prop3DAcoIsoDenQ_DEO2_FDTD.h:351-498)
+static inline bool iso_inj_vb(Prop *p, float *dV, float *dB, float *P, float *DP) {
+    if (!p->begin(bit(F_V) | bit(F_B) | bit(F_PCUR))) return false;
+    Prop::Arg a[4];
+    int na = 0;
+    if (dV) WFD_ARG(dV, 0, true, false, a[na++]);
+    Prop::Arg &adB = a[na];
+    WFD_ARG(dB, 1, true, false, a[na++]);
+    Prop::Arg &aP = a[na];
+    WFD_ARG(P, 2, true, false, a[na++]);
+    Prop::Arg &aDP = a[na];
+    WFD_ARG(DP, 3, true, false, a[na++]);
+    float *t[6];
+    for (int i = 0; i < 6; i++)
+        if (!(t[i] = p->scratch(i))) return false;
+    if (!dplus(p, aP.dev, t[0], t[1], t[2])) return false;
+    k_scale3<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, adB.dev, F3{t[0], t[1], t[2]}, W3{t[3], t[4], t[5]});
+    WFD_LAUNCHED();
+    if (!dminus(p, t[3], t[4], t[5], t[0], t[1], t[2])) return false;
+    k_iso_inj_vb<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, p->f[F_V].d, p->f[F_B].d,
+                                                            dV ? a[0].dev : nullptr, adB.dev, aDP.dev,
+                                                            F3{t[0], t[1], t[2]}, p->f[F_PCUR].d);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, na)) return false;
+    return p->end(bit(F_PCUR));
+}
+
+static inline bool vti_inj_v(Prop *p, float *dV, float *DP, float *DM) {
+    if (!p->begin(bit(F_V) | bit(F_PCUR) | bit(F_MCUR))) return false;
+    Prop::Arg a[3];
+    WFD_ARG(dV, 0, true, false, a[0]);
+    WFD_ARG(DP, 1, true, false, a[1]);
+    WFD_ARG(DM, 2, true, false, a[2]);
+    k_vti_inj_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, p->f[F_V].d, a[0].dev, a[1].dev, a[2].dev,
+                                                           p->f[F_PCUR].d, p->f[F_MCUR].d);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, 3)) return false;
+    return p->end(bit(F_PCUR) | bit(F_MCUR));
+}
+
+// VEA source (src/prop3DAcoVTIDenQ_DEO2_FDTD.h:457-551, src/prop3DAcoTTIDenQ_DEO2_FDTD.h:460-554)
+static inline bool vea_inj(Prop *p, float *dV, float *dE, float *dA, float *P, float *M, float *DP, float *DM) {
+    if (!p->begin(p->model_mask() | bit(F_PCUR) | bit(F_MCUR))) return false;
+    Prop::Arg a[7];
+    WFD_ARG(dV, 0, true, false, a[0]);
+    WFD_ARG(dE, 1, true, false, a[1]);
+    WFD_ARG(dA, 2, true, false, a[2]);
+    WFD_ARG(P, 3, true, false, a[3]);
+    WFD_ARG(M, 4, true, false, a[4]);
+    WFD_ARG(DP, 5, true, false, a[5]);
+    WFD_ARG(DM, 6, true, false, a[6]);
+    float *t[12];
+    for (int i = 0; i < 12; i++)
+        if (!(t[i] = p->scratch(i))) return false;
+    if (!rplus(p, a[3].dev, t[0], t[1], t[2])) return false;
+    if (!rplus(p, a[4].dev, t[3], t[4], t[5])) return false;
+    k_vea_inj_mid<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->f[F_B].d, p->f[F_ETA].d, p->f[F_F].d, a[1].dev,
+                                                             a[2].dev, F3{t[0], t[1], t[2]}, F3{t[3], t[4], t[5]},
+                                                             W3{t[6], t[7], t[8]}, W3{t[9], t[10], t[11]});
+    WFD_LAUNCHED();
+    if (!rminus(p, t[6], t[7], t[8], t[0], t[1], t[2])) return false;
+    if (!rminus(p, t[9], t[10], t[11], t[3], t[4], t[5])) return false;
+    k_vea_inj_final<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, p->f[F_V].d, p->f[F_B].d, a[0].dev,
+                                                               a[5].dev, a[6].dev, F3{t[0], t[1], t[2]},
+                                                               F3{t[3], t[4], t[5]}, p->f[F_PCUR].d, p->f[F_MCUR].d);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, 7)) return false;
+    return p->end(bit(F_PCUR) | bit(F_MCUR));
+}
+
+// ---------------------------------------------------------------------------------------
+// Born imaging conditions
+// ---------------------------------------------------------------------------------------
+
+static inline bool iso_acc_v(Prop *p, float *gV, float *DP) {
+    if (!p->begin(bit(F_V) | bit(F_B) | bit(F_POLD))) return false;
+    Prop::Arg a[2];
+    WFD_ARG(gV, 0, true, true, a[0]);
+    WFD_ARG(DP, 1, true, false, a[1]);
+    k_iso_acc_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->f[F_V].d, p->f[F_B].d, a[1].dev, p->f[F_POLD].d,
+                                                           a[0].dev);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, 2)) return false;
+    return p->end(0);
+}
+
+// VB (gV != null) and B imaging (src/prop3DAcoIsoDenQ_DEO2_FDTD.h:563-656)
+static inline bool iso_acc_vb(Prop *p, float *gV, float *gB, float *P, float *DP) {
+    if (!p->begin(bit(F_V) | bit(F_B) | bit(F_POLD))) return false;
+    Prop::Arg a[4];
+    int na = 0;
+    if (gV) WFD_ARG(gV, 0, true, true, a[na++]);
+    Prop::Arg &agB = a[na];
+    WFD_ARG(gB, 1, true, true, a[na++]);
+    Prop::Arg &aP = a[na];
+    WFD_ARG(P, 2, true, false, a[na++]);
+    Prop::Arg &aDP = a[na];
+    WFD_ARG(DP, 3, true, false, a[na++]);
+    float *t[6];
+    for (int i = 0; i < 6; i++)
+        if (!(t[i] = p->scratch(i))) return false;
+    if (!dplus(p, aP.dev, t[0], t[1], t[2])) return false;
+    if (!dplus(p, p->f[F_POLD].d, t[3], t[4], t[5])) return false;
+    k_iso_acc_vb<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->f[F_V].d, p->f[F_B].d, aDP.dev, p->f[F_POLD].d,
+                                                            F3{t[0], t[1], t[2]}, F3{t[3], t[4], t[5]},
+                                                            gV ? a[0].dev : nullptr, agB.dev);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, na)) return false;
+    return p->end(0);
+}
+
+static inline bool vti_acc_v(Prop *p, float *gV, float *DP, float *DM) {
+    if (!p->begin(bit(F_V) | bit(F_B) | bit(F_POLD) | bit(F_MOLD))) return false;
+    Prop::Arg a[3];
+    WFD_ARG(gV, 0, true, true, a[0]);
+    WFD_ARG(DP, 1, true, false, a[1]);
+    WFD_ARG(DM, 2, true, false, a[2]);
+    k_vti_acc_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->f[F_V].d, p->f[F_B].d, a[1].dev, a[2].dev,
+                                                           p->f[F_POLD].d, p->f[F_MOLD].d, a[0].dev);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, 3)) return false;
+    return p->end(0);
+}
+
+// VEA imaging (src/prop3DAcoVTIDenQ_DEO2_FDTD.h:730-787, src/prop3DAcoTTIDenQ_DEO2_FDTD.h:733-790)
+static inline bool vea_acc(Prop *p, float *gV, float *gE, float *gA, float *P, float *M, float *DP, float *DM) {
+    if (!p->begin(p->model_mask() | bit(F_POLD) | bit(F_MOLD))) return false;
+    Prop::Arg a[7];
+    WFD_ARG(gV, 0, true, true, a[0]);
+    WFD_ARG(gE, 1, true, true, a[1]);
+    WFD_ARG(gA, 2, true, true, a[2]);
+    WFD_ARG(P, 3, true, false, a[3]);
+    WFD_ARG(M, 4, true, false, a[4]);
+    WFD_ARG(DP, 5, true, false, a[5]);
+    WFD_ARG(DM, 6, true, false, a[6]);
+    float *t[12];
+    for (int i = 0; i < 12; i++)
+        if (!(t[i] = p->scratch(i))) return false;
+    if (!rplus(p, a[3].dev, t[0], t[1], t[2])) return false;
+    if (!rplus(p, a[4].dev, t[3], t[4], t[5])) return false;
+    if (!rplus(p, p->f[F_POLD].d, t[6], t[7], t[8])) return false;
+    if (!rplus(p, p->f[F_MOLD].d, t[9], t[10], t[11])) return false;
+    k_vea_acc<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(
+        p->n, p->f[F_V].d, p->f[F_B].d, p->f[F_ETA].d, p->f[F_F].d, a[5].dev, a[6].dev, p->f[F_POLD].d,
+        p->f[F_MOLD].d, F3{t[0], t[1], t[2]}, F3{t[3], t[4], t[5]}, F3{t[6], t[7], t[8]}, F3{t[9], t[10], t[11]},
+        a[0].dev, a[1].dev, a[2].dev);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, 7)) return false;
+    return p->end(0);
+}
+
+// ---------------------------------------------------------------------------------------
+// wavefield-separation imaging (src/prop3DAcoIsoDenQ_DEO2_FDTD.h:673-765,
+// src/prop3DAcoVTIDenQ_DEO2_FDTD.h:606-725): per (x,y) column, zero-pad to 2nz, transform
+// with sign +1, zero half the spectrum, transform back with sign -1, g += 2b/v^3 Re(S R).
+// The transforms are batched 1D C2C FFTs over a chunk of columns (hipFFT; FFTW's sign +1
+// is hipFFT's BACKWARD direction, both unnormalised).
+// ---------------------------------------------------------------------------------------
+
+__global__ void k_sep_pack(long ncol, long nz, long nfft, long col0, float scale, const float *__restrict__ a,
+                           float2 *__restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncol * nfft) return;
+    const long c = i / nfft, j = i - c * nfft;
+    out[i] = j < nz ? make_float2(scale * a[(col0 + c) * nz + j], 0.0f) : make_float2(0.0f, 0.0f);
+}
+
+// zero bins lo+1 .. lo+nfft/2-1 of every column
+__global__ void k_sep_filter(long ncol, long nfft, long lo, float2 *__restrict__ s) {
+    const long half = nfft / 2;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncol * (half - 1)) return;
+    const long c = i / (half - 1), j = i - c * (half - 1) + 1;
+    s[c * nfft + lo + j] = make_float2(0.0f, 0.0f);
+}
+
+__global__ void k_sep_accum(long ncol, long nz, long nfft, long col0, const float *__restrict__ v,
+                            const float *__restrict__ b, const float2 *__restrict__ s1, const float2 *__restrict__ r1,
+                            const float2 *__restrict__ s2, const float2 *__restrict__ r2, float *__restrict__ g) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncol * nz) return;
+    const long c = i / nz, j = i - c * nz;
+    const long k = (col0 + c) * nz + j;
+    const float V = v[k], B = b[k];
+    const float factor = 2 * B / (V * V * V);
+    const float2 a = s1[c * nfft + j], r = r1[c * nfft + j];
+    float re = a.x * r.x - a.y * r.y;
+    if (s2) {
+        const float2 a2 = s2[c * nfft + j], q2 = r2[c * nfft + j];
+        re = re + (a2.x * q2.x - a2.y * q2.y);
+    }
+    g[k] += factor * re;
+}
+
+struct SepPlan {
+    hipfftHandle plan = 0;
+    long ncol = 0, nfft = 0;
+    float2 *buf[4] = {};
+    ~SepPlan() {
+        if (plan) hipfftDestroy(plan);
+        for (auto &b : buf)
+            if (b) hipFree(b);
+    }
+};
+
+static inline bool wavefield_sep(Prop *p, float *gV, float *DP, float *DM, long isFWI) {
+    const bool two = p->phys != ISO;
+    if (!p->begin(bit(F_V) | bit(F_B) | bit(F_POLD) | (two ? bit(F_MOLD) : 0u))) return false;
+    Prop::Arg a[3];
+    WFD_ARG(gV, 0, true, true, a[0]);
+    WFD_ARG(DP, 1, true, false, a[1]);
+    if (two) WFD_ARG(DM, 2, true, false, a[2]);
+    const long nz = p->nz, nfft = 2 * nz, ncols = p->nx * p->ny;
+    // chunk so each complex buffer stays near 128 MiB
+    long chunk = std::max<long>(1, (128l << 20) / (long)(sizeof(float2) * nfft));
+    chunk = std::min(chunk, ncols);
+    SepPlan sp;
+    int nfi = (int)nfft;
+    if (hipfftPlanMany(&sp.plan, 1, &nfi, nullptr, 1, nfi, nullptr, 1, nfi, HIPFFT_C2C, (int)chunk) != HIPFFT_SUCCESS) {
+        set_error("hipfftPlanMany failed");
+        return false;
+    }
+    hipfftSetStream(sp.plan, p->stream);
+    const int nbuf = two ? 4 : 2;
+    for (int i = 0; i < nbuf; i++) WFD_TRY(hipMalloc(&sp.buf[i], sizeof(float2) * chunk * nfft));
+    const float scale = 1.0f / (float)(nfft);
+    const long kadj = isFWI ? 0 : nfft / 2;
+    const float *src[4] = {a[1].dev, p->f[F_POLD].d, two ? a[2].dev : nullptr, two ? p->f[F_MOLD].d : nullptr};
+    for (long col0 = 0; col0 < ncols; col0 += chunk) {
+        const long nc = std::min(chunk, ncols - col0);
+        for (int i = 0; i < nbuf; i++) {
+            k_sep_pack<<<(unsigned)((nc * nfft + 255) / 256), 256, 0, p->stream>>>(nc, nz, nfft, col0, scale, src[i],
+                                                                                   sp.buf[i]);
+            WFD_LAUNCHED();
+            if (hipfftExecC2C(sp.plan, (hipfftComplex *)sp.buf[i], (hipfftComplex *)sp.buf[i], HIPFFT_BACKWARD) !=
+                HIPFFT_SUCCESS) {
+                set_error("hipfftExecC2C failed");
+                return false;
+            }
+            // nonlinear (even i) keeps bins 0..nz; adjoint (odd i) zeroes per FWI/RTM
+            const long lo = (i % 2 == 0) ? nfft / 2 : kadj;
+            if (nfft / 2 > 1) {
+                k_sep_filter<<<(unsigned)((nc * (nfft / 2 - 1) + 255) / 256), 256, 0, p->stream>>>(nc, nfft, lo,
+                                                                                                   sp.buf[i]);
+                WFD_LAUNCHED();
+            }
+            if (hipfftExecC2C(sp.plan, (hipfftComplex *)sp.buf[i], (hipfftComplex *)sp.buf[i], HIPFFT_FORWARD) !=
+                HIPFFT_SUCCESS) {
+                set_error("hipfftExecC2C failed");
+                return false;
+            }
+        }
+        k_sep_accum<<<(unsigned)((nc * nz + 255) / 256), 256, 0, p->stream>>>(
+            nc, nz, nfft, col0, p->f[F_V].d, p->f[F_B].d, sp.buf[0], sp.buf[1], two ? sp.buf[2] : nullptr,
+            two ? sp.buf[3] : nullptr, a[0].dev);
+        WFD_LAUNCHED();
+    }
+    WFD_TRY(hipStreamSynchronize(p->stream));
+    if (!p->finish_args(a, two ? 3 : 2)) return false;
+    return p->end(0);
+}
+
+}  // namespace wfd
