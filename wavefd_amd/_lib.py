@@ -90,10 +90,20 @@ SPACETIME = {
 }
 
 
+CODEC = {
+    "wavefd_codec_words": (_L, [_P, _P, _L]),
+    "wavefd_codec_compress": (_L, [_P, _P, _P, _L, _P]),
+    "wavefd_codec_decompress": (_L, [_P, _L, _P]),
+    "wavefd_codec_LastError": (ctypes.c_char_p, []),
+}
+
+
 def signatures(kind):
-    """{symbol: (restype, argtypes)} for 'iso' | 'vti' | 'tti' | 'spacetime'."""
+    """{symbol: (restype, argtypes)} for 'iso' | 'vti' | 'tti' | 'spacetime' | 'codec'."""
     if kind == "spacetime":
         return dict(SPACETIME)
+    if kind == "codec":
+        return dict(CODEC)
     pre = PREFIXES[kind]
     out = {}
     for name, sig in list(_COMMON.items()) + list(_PHYS[kind].items()):
@@ -104,6 +114,8 @@ def signatures(kind):
 def lib_path(kind):
     if kind == "spacetime":
         name = "libspacetime.so"
+    elif kind == "codec":
+        name = "libwavefd_codec.so"
     else:  # the reference's library names start in lower case: libprop3DAco...
         pre = PREFIXES[kind]
         name = "lib" + pre[0].lower() + pre[1:] + ".so"
@@ -119,7 +131,7 @@ class _Bound:
     def __init__(self, kind, dll):
         self.kind = kind
         self.dll = dll
-        self.prefix = "" if kind == "spacetime" else PREFIXES[kind] + "_"
+        self.prefix = PREFIXES[kind] + "_" if kind in PREFIXES else ""
 
     def __getattr__(self, name):
         fn = getattr(self.dll, self.prefix + name)
@@ -147,5 +159,10 @@ def load(kind):
 
 def last_error(kind):
     lib = load(kind)
-    msg = lib.spacetime_LastError() if kind == "spacetime" else lib.LastError()
+    if kind == "spacetime":
+        msg = lib.spacetime_LastError()
+    elif kind == "codec":
+        msg = lib.wavefd_codec_LastError()
+    else:
+        msg = lib.LastError()
     return msg.decode() if msg else ""
