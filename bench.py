@@ -203,6 +203,18 @@ def cpu_baseline(physics, n, target_s=12.0):
                 physics.upper(), n, k, dt)}
 
 
+def copy_probe(nbytes=1 << 30, reps=3):
+    """A plain copy kernel of known size: reads and writes nbytes per launch."""
+    import torch
+    src = torch.rand(nbytes // 4, device="cuda", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "launches": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,7 +226,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true", help="also time Iso 256^3 and VTI 512^3")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy kernel 3 times (the known-size "
+                         "kernel that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the propagator state of one shot lives on one MI355X and this "
               "build has no multi-GPU path; run with --gpus 1.", file=sys.stderr)

@@ -3,308 +3,444 @@
 // (e.g. src/prop3DAcoTTIDenQ_DEO2_FDTD.h:359-379): the D+ temporaries never reach HBM.
 //
 // Decomposition ("2.5D", marching along x, the slowest axis):
-//   * a workgroup owns an output tile of TZO z-cells x TYO y-cells and a chunk of CX x-planes;
-//   * its 64 x (TYO+7) threads cover the tile plus the 4-cell halo (-4..+3) that the
-//     backward difference D- needs, lanes along z (coalesced), one row of y per wavefront;
-//   * every thread keeps, in registers, the last 8 x-planes of p (and m) of its column and
-//     the last 8 x-planes of the x-temporaries; the temporaries along y and z are exchanged
-//     through LDS;
-//   * at iteration x a thread loads p(x+7), computes the full gradient and the material
-//     sandwich at plane x+3 (D+y, D+z from an LDS copy of plane x+3 with a 7-cell border),
-//     and the output threads then apply D- and the leapfrog update at plane x.
+//   * a workgroup of ZW x NROW threads owns a (y,z) tile of temporaries, ZW cells along z
+//     (lanes, coalesced) by NROW rows along y; its outputs are the inner (ZW-7) x (NROW-7)
+//     cells (the backward difference D- reads temporaries at offsets -4..+3), and it marches
+//     over a chunk of x-planes;
+//   * every thread keeps, in registers, p(x..x+7) of its column (the D+x operand) and the
+//     x-temporaries of planes x-4..x+3 (the D-x operand);
+//   * at step x the workgroup stages plane x+3 of p (and m) in LDS with a 7-cell border,
+//     computes the full gradient and the material sandwich at (x+3, y, z) for the whole tile,
+//     keeps the y/z temporaries in a 4-plane LDS ring, and the output threads then apply D-
+//     and the leapfrog update at plane x.
 // Halo threads recompute the sandwich of their neighbours' cells, which is what lets TTI's
 // rotated sandwich (full gradient at every temporary) fuse at all.
+//
+// Memory: every global access is a uniform plane base plus a 32-bit byte offset inside the
+// plane computed once per thread; out-of-grid planes and columns are clamped for the address
+// and zeroed, so no access needs a bounds branch.  Every
+// operand of step x+1 (p(x+8), the border of plane x+4, the material at x+4, v/q/pOld at x+1)
+// is requested before step x's barrier, one step ahead of its use.  One barrier per step:
+// the p-plane buffer is double-buffered and the temporaries live in a 4-plane ring.
+// Workgroups are numbered so that each XCD (workgroup id mod 8) sweeps a contiguous run of
+// tiles, which keeps neighbouring tiles, whose halos overlap, in the same L2.
 //
 // Semantics match the operator kernels exactly: temporaries are 0 outside the update
 // region (the reference's zeroed annulus), the free surface uses the odd mirror for D+ and
 // the even mirror for D-, x/y derivatives vanish at kz=0 and L=0 there.  Expression order
 // is unchanged, so results are bit-identical to the two-pass path.
 #pragma once
+#include <type_traits>
+
 #include "wfd_kernels.hpp"
 #include "wfd_runtime.hpp"
 
 namespace wfd {
 
-constexpr int FZ = 64;        // threads along z per row (one wavefront)
-constexpr int FZO = 56;       // output z cells per tile (lanes 4..59)
-constexpr int FYO = 8;        // output y rows per tile
-constexpr int FYE = FYO + 7;  // rows of threads (tile + halo)
-constexpr int FLZ = FZ + 7;   // LDS plane width along z: lanes -3 .. 67 around the E lanes
-constexpr int FLY = FYE + 7;  // LDS plane rows
-
 template <int PHYS>
 struct FusedArgs {
     Grid g;
     float dt2;
-    long cx;  // x-planes per chunk
+    int cx;        // x-planes per chunk
+    int nzt, nyt;  // tiles along z and y
+    int bz, by;    // tiles per block of the XCD-local order (bz = 0: plain z-fastest order)
     const float *pCur, *mCur;
     float *pOld, *mOld, *pSpace, *mSpace;
     const float *v, *b, *q, *eps, *eta, *f, *sT, *cT, *sP, *cP;
 };
+
+template <int NF, int ZW, int NROW>
+struct FusedLds {
+    static constexpr int LZ = ZW + 7, LY = NROW + 7, YO = NROW - 7;
+    float sp[2][NF][LY * LZ];     // p (and m) of plane x+3 with a 7-cell border, double-buffered
+    float soy[4][NF][NROW * ZW];  // y-temporaries, ring of planes x .. x+3
+    float soz[4][NF][YO * ZW];    // z-temporaries of the output rows, same ring
+};
+
+// Global accesses are global_load/store with a uniform plane base in SGPRs (saddr) and the
+// 32-bit byte offset of the column inside the plane in a VGPR.  The offsets are passed
+// through an empty asm once per step so that the compiler keeps their zero extension next to
+// the accesses (hoisted out of the loop it would become a 64-bit VGPR address per field).  Plane
+// indices outside [0, nx) are clamped for the address and their values replaced by 0;
+// columns outside the grid use offset 0 and are zeroed the same way, so every address stays
+// inside the field.
+__device__ __forceinline__ const char *plane(const float *a, int xx, int nx, long nynz) {
+    const int xc = xx < 0 ? 0 : (xx >= nx ? nx - 1 : xx);
+    return reinterpret_cast<const char *>(a + (long)xc * nynz);
+}
+__device__ __forceinline__ float gld(const char *base, uint32_t off) {
+    return *reinterpret_cast<const float *>(base + off);
+}
+__device__ __forceinline__ void gst(float v, float *a, int xx, long nynz, uint32_t off) {
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(a + (long)xx * nynz) + off) = v;
+}
+__device__ __forceinline__ uint32_t launder(uint32_t off) {
+    asm volatile("" : "+v"(off));
+    return off;
+}
+__device__ __forceinline__ float flip(float v, uint32_t sign) {
+    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, v) ^ sign);
+}
+
+// byte offset of column (yy, zz) inside a plane; above a free surface the column of the odd
+// mirror p[-j] = -p[j] with sign = the sign bit; ok = false outside the grid (offset 0)
+__device__ __forceinline__ uint32_t col_off(int yy, int zz, int ny, int nz, bool mirror, uint32_t &sign,
+                                            bool &ok) {
+    sign = 0;
+    ok = false;
+    if (yy < 0 || yy >= ny) return 0;
+    if (zz >= 0 && zz < nz) {
+        ok = true;
+        return (uint32_t)(yy * nz + zz) * 4u;
+    }
+    if (mirror && zz < 0 && -zz < nz) {
+        sign = 0x80000000u;
+        ok = true;
+        return (uint32_t)(yy * nz - zz) * 4u;
+    }
+    return 0;
+}
 
 __device__ __forceinline__ float sten8(const float *a) {
     // a[0..7] = values at offsets -3..+4 (D+) or -4..+3 (D-) of the centre pair a[3],a[4]
     return C1 * (-a[3] + a[4]) + C2 * (-a[2] + a[5]) + C3 * (-a[1] + a[6]) + C4 * (-a[0] + a[7]);
 }
 
-template <int PHYS>
-__global__ void __launch_bounds__(FZ *FYE) k_fused_step(FusedArgs<PHYS> A) {
+template <int N>
+using IC = std::integral_constant<int, N>;
+
+// FST: this tile holds the free-surface rows (fs and the first z-tile)
+template <int PHYS, int ZW, int NROW, bool FST>
+__device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PHYS == ISO ? 1 : 2, ZW, NROW> &S,
+                                           int zt, int yt, int xc) {
     constexpr int NF = PHYS == ISO ? 1 : 2;
+    constexpr int NM = PHYS == ISO ? 1 : (PHYS == VTI ? 4 : 8);  // material fields of the sandwich
+    constexpr int NO = 2 + NF;                                   // v, q, pOld (, mOld)
+    constexpr int ZO = ZW - 7, YO = NROW - 7, LZ = ZW + 7;
+    constexpr int NB = 7 * (LZ + NROW);  // border cells of the staged plane
+    static_assert(NB <= ZW * NROW, "one border cell per thread at most");
     const Grid &g = A.g;
-    const int l = threadIdx.x;  // lane
-    const int r = threadIdx.y;  // row
-    const long z0 = (long)blockIdx.x * FZO;
-    const long y0 = (long)blockIdx.y * FYO;
-    const long xs = (long)blockIdx.z * A.cx;
-    const long xe = xs + A.cx < g.nx ? xs + A.cx : g.nx;
-    const long z = z0 - 4 + l;
-    const long y = y0 - 4 + r;
-    const bool zin = z >= 0 && z < g.nz, yin = y >= 0 && y < g.ny;
-    const bool col_ok = zin && yin;
-    const long zlo = g.fs ? 0 : 4;
-    const bool yzU = y >= 4 && y < g.ny - 4 && z >= zlo && z < g.nz - 4;  // (y,z) inside the update region
-    const bool out_thread = l >= 4 && l < 4 + FZO && r >= 4 && r < 4 + FYO && col_ok;
-    const long colk = y * g.nz + z;  // offset inside a plane (valid only when col_ok)
-
-    __shared__ float sp[NF][FLY][FLZ];      // p (and m) at plane x+3, border 7
-    __shared__ float soy[NF][FYE][FZ];      // y-temporaries at plane x, E rows
-    __shared__ float soz[NF][FYO][FZ + 1];  // z-temporaries at plane x, output rows
-
+    const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
+    const int t = threadIdx.x, l = t % ZW, r = t / ZW;
+    const int z0 = zt * ZO, y0 = yt * YO;
+    const int z = z0 - 4 + l, y = y0 - 4 + r;
+    const int xs = xc * A.cx;
+    const int xe = xs + A.cx < nx ? xs + A.cx : nx;
+    const int zlo = FST ? 0 : 4;
+    const bool yzU = y >= 4 && y < ny - 4 && z >= zlo && z < nz - 4;
+    const bool out_thread = l >= 4 && l < 4 + ZO && r >= 4 && r < 4 + YO && y < ny && z < nz;
+    uint32_t own_sign;
+    bool own_ok;
+    const uint32_t own = col_off(y, z, ny, nz, FST, own_sign, own_ok);
+    const uint32_t moff = yzU ? own : 0u;  // material: only used inside the update region
+    const uint32_t ooff = out_thread ? own : 0u;
+    // this thread's border cell of the staged plane, if any
+    uint32_t boff = 0, bsign = 0;
+    bool b_ok = false;
+    int blds = 0;
+    const bool has_b = t < NB;
+    if (has_b) {
+        int sy, sz;
+        if (t < 3 * LZ) {
+            sy = t / LZ;
+            sz = t - sy * LZ;
+        } else if (t < 3 * LZ + 7 * NROW) {
+            const int m = t - 3 * LZ, c = m % 7;
+            sy = 3 + m / 7;
+            sz = c < 3 ? c : ZW + c;
+        } else {
+            const int m = t - 3 * LZ - 7 * NROW;
+            sy = 3 + NROW + m / LZ;
+            sz = m - (sy - 3 - NROW) * LZ;
+        }
+        boff = col_off(y0 - 7 + sy, z0 - 7 + sz, ny, nz, FST, bsign, b_ok);
+        blds = sy * LZ + sz;
+    }
     const float *fld[2] = {A.pCur, A.mCur};
-    float pq[NF][8];   // p(x .. x+7) of this column
-    float oxq[NF][8];  // x-temporaries at planes x-4 .. x+3
-    float ry[NF][4], rz[NF][4];  // y/z temporaries of planes x .. x+3 (ring, slot = plane & 3)
-    float bq[4];                 // b at planes x .. x+3 (slot = plane & 3)
+    const float *mat[8] = {A.b, A.eps, A.eta, A.f, A.sT, A.cT, A.sP, A.cP};
+    const float *oin[4] = {A.v, A.q, A.pOld, A.mOld};
 
-    auto ld = [&](const float *a, long xx) -> float {
-        return (col_ok && xx >= 0 && xx < g.nx) ? a[xx * g.nynz + colk] : 0.0f;
+    const long nynz = g.nynz;
+    // loads are issued raw; the zeroing outside the grid and the mirror's sign are applied
+    // when the value is consumed, so that no wait follows a load
+    auto ld = [&](const float *a, int xx, uint32_t off) -> float { return gld(plane(a, xx, nx, nynz), off); };
+    uint32_t own_s, boff_s, moff_s, ooff_s;  // this step's copies of the offsets
+    auto launder_all = [&]() {
+        own_s = launder(own);
+        boff_s = launder(boff);
+        moff_s = launder(moff);
+        ooff_s = launder(ooff);
+    };
+    auto fixp = [&](float v, int xx) -> float {
+        v = (own_ok && xx >= 0 && xx < nx) ? v : 0.0f;
+        if constexpr (FST) v = flip(v, own_sign);
+        return v;
+    };
+    auto fixb = [&](float v, int xx) -> float {
+        v = (b_ok && xx >= 0 && xx < nx) ? v : 0.0f;
+        if constexpr (FST) v = flip(v, bsign);
+        return v;
     };
 
+    float pq[NF][8], pn[NF], oxq[NF][8], bq[4], bn[NF];
+    float mc[NM], mnx[NM], oc[NO], onx[NO];
+    const int xfirst = xs - 7;
 #pragma unroll
     for (int fi = 0; fi < NF; fi++) {
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            pq[fi][j] = 0.0f;
-            oxq[fi][j] = 0.0f;
-        }
+        for (int j = 0; j < 8; j++) oxq[fi][j] = 0.0f;
+        pq[fi][0] = 0.0f;
+    }
+    launder_all();
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            ry[fi][j] = 0.0f;
-            rz[fi][j] = 0.0f;
-        }
+    for (int fi = 0; fi < NF; fi++) {
+#pragma unroll
+        for (int j = 1; j < 8; j++) pq[fi][j] = fixp(ld(fld[fi], xfirst + j - 1, own_s), xfirst + j - 1);
+        pn[fi] = ld(fld[fi], xfirst + 7, own_s);
+        bn[fi] = ld(fld[fi], xfirst + 3, boff_s);
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) bq[j] = 0.0f;
-    // preload p(x0-7 .. x0-1) into slots 1..7 so the first iteration shifts them to 0..6
-    const long xfirst = xs - 7;
 #pragma unroll
-    for (int fi = 0; fi < NF; fi++)
+    for (int k = 0; k < NM; k++) mnx[k] = ld(mat[k], xfirst + 3, moff_s);
 #pragma unroll
-        for (int j = 1; j < 8; j++) pq[fi][j] = ld(fld[fi], xfirst + j - 1);
+    for (int k = 0; k < NO; k++) onx[k] = ld(oin[k], xfirst, ooff_s);
 
-    for (long x = xfirst; x < xe; x++) {
-        const long xp = x + 3;  // plane whose temporaries are produced this iteration
-        // ---- shift queues, load p(x+7) ------------------------------------------------
+    auto iter = [&](auto U, int x) {
+        constexpr int u = decltype(U)::value;  // x - xfirst = u (mod 4)
+        constexpr int s2 = (u + 3) & 1;        // p-plane buffer of plane x+3
+        constexpr int sw = (u + 3) & 3;        // ring slot of plane x+3 (written now)
+        constexpr int sr = u & 3;              // ring slot of plane x (read now)
+        // ---- 1. shift the queues, stage plane x+3 in LDS --------------------------------
 #pragma unroll
         for (int fi = 0; fi < NF; fi++) {
 #pragma unroll
-            for (int j = 0; j < 7; j++) {
-                pq[fi][j] = pq[fi][j + 1];
-                oxq[fi][j] = oxq[fi][j + 1];
-            }
-            pq[fi][7] = ld(fld[fi], x + 7);
+            for (int j = 0; j < 7; j++) pq[fi][j] = pq[fi][j + 1];
+            pq[fi][7] = fixp(pn[fi], x + 7);
+            S.sp[s2][fi][(r + 3) * LZ + l + 3] = pq[fi][3];
+            if (has_b) S.sp[s2][fi][blds] = fixb(bn[fi], x + 3);
         }
-        const bool xpU = xp >= 4 && xp < g.nx - 4;
-        const bool inU = xpU && yzU;  // (x+3, y, z) in the update region
-        // ---- stage plane x+3 of p (and m) in LDS with a 7-cell border -------------------
-        __syncthreads();  // previous iteration's readers are done
+#pragma unroll
+        for (int k = 0; k < NM; k++) mc[k] = mnx[k];
+#pragma unroll
+        for (int k = 0; k < NO; k++) oc[k] = onx[k];
+        // ---- 2. request the operands of step x+1 ---------------------------------------
+        launder_all();
 #pragma unroll
         for (int fi = 0; fi < NF; fi++) {
-            float val = pq[fi][3];
-            if (g.fs && z < 0 && yin && xp >= 0 && xp < g.nx && -z < g.nz)
-                val = -fld[fi][xp * g.nynz + y * g.nz - z];  // odd mirror above the surface
-            sp[fi][r + 3][l + 3] = val;
+            pn[fi] = ld(fld[fi], x + 8, own_s);
+            bn[fi] = ld(fld[fi], x + 4, boff_s);
         }
-        for (int s = r * FZ + l; s < FLY * FLZ; s += FZ * FYE) {
-            const int sy = s / FLZ, sz = s - sy * FLZ;
-            if (sy >= 3 && sy < 3 + FYE && sz >= 3 && sz < 3 + FZ) continue;  // owned by a thread
-            const long yy = y0 - 7 + sy, zz = z0 - 7 + sz;
-            float vals[NF];
 #pragma unroll
-            for (int fi = 0; fi < NF; fi++) vals[fi] = 0.0f;
-            if (yy >= 0 && yy < g.ny && xp >= 0 && xp < g.nx) {
-                if (zz >= 0 && zz < g.nz) {
+        for (int k = 0; k < NM; k++) mnx[k] = ld(mat[k], x + 4, moff_s);
 #pragma unroll
-                    for (int fi = 0; fi < NF; fi++) vals[fi] = fld[fi][xp * g.nynz + yy * g.nz + zz];
-                } else if (g.fs && zz < 0 && -zz < g.nz) {
-#pragma unroll
-                    for (int fi = 0; fi < NF; fi++) vals[fi] = -fld[fi][xp * g.nynz + yy * g.nz - zz];
-                }
-            }
-#pragma unroll
-            for (int fi = 0; fi < NF; fi++) sp[fi][sy][sz] = vals[fi];
-        }
+        for (int k = 0; k < NO; k++) onx[k] = ld(oin[k], x + 1, ooff_s);
         __syncthreads();
-        // ---- gradient + material sandwich at (x+3, y, z) --------------------------------
+        // ---- 3. gradient + material sandwich at (x+3, y, z) ----------------------------
+        const int xp = x + 3;
         float ox[NF], oy[NF], oz[NF];
 #pragma unroll
         for (int fi = 0; fi < NF; fi++) ox[fi] = oy[fi] = oz[fi] = 0.0f;
-        if (inU) {
+        if (xp >= 4 && xp < nx - 4 && yzU) {
             float dx[NF], dy[NF], dz[NF];
 #pragma unroll
             for (int fi = 0; fi < NF; fi++) {
                 float a[8];
 #pragma unroll
-                for (int j = 0; j < 8; j++) a[j] = sp[fi][r + j][l + 3];  // rows y-3 .. y+4
+                for (int j = 0; j < 8; j++) a[j] = S.sp[s2][fi][(r + j) * LZ + l + 3];  // rows y-3 .. y+4
                 dy[fi] = g.idy * sten8(a);
 #pragma unroll
-                for (int j = 0; j < 8; j++) a[j] = sp[fi][r + 3][l + j];  // lanes z-3 .. z+4
+                for (int j = 0; j < 8; j++) a[j] = S.sp[s2][fi][(r + 3) * LZ + l + j];  // lanes z-3 .. z+4
                 dz[fi] = g.idz * sten8(a);
                 dx[fi] = g.idx * sten8(pq[fi]);
-                if (z == 0) {  // free-surface row: x/y derivatives vanish
+                if (FST && z == 0) {  // free-surface row: x/y derivatives vanish
                     dx[fi] = 0.0f;
                     dy[fi] = 0.0f;
                 }
             }
-            const long k = xp * g.nynz + colk;
             if constexpr (PHYS == ISO) {
-                const float B = A.b[k];
-                bq[xp & 3] = B;
-                ox[0] = B * dx[0];
-                oy[0] = B * dy[0];
-                oz[0] = B * dz[0];
+                ox[0] = mc[0] * dx[0];
+                oy[0] = mc[0] * dy[0];
+                oz[0] = mc[0] * dz[0];
             } else if constexpr (PHYS == VTI) {
-                const float B = A.b[k];
-                bq[xp & 3] = B;
-                vti_material(dx[0], dy[0], dz[0], dx[1], dy[1], dz[1], A.eps[k], A.eta[k], A.f[k], B, ox[0], oy[0],
+                vti_material(dx[0], dy[0], dz[0], dx[1], dy[1], dz[1], mc[1], mc[2], mc[3], mc[0], ox[0], oy[0],
                              oz[0], ox[1], oy[1], oz[1]);
             } else {
-                const float B = A.b[k];
-                bq[xp & 3] = B;
-                tti_material(dx[0], dy[0], dz[0], dx[1], dy[1], dz[1], A.sT[k], A.cT[k], A.sP[k], A.cP[k], A.eps[k],
-                             A.eta[k], A.f[k], B, ox[0], oy[0], oz[0], ox[1], oy[1], oz[1]);
+                tti_material(dx[0], dy[0], dz[0], dx[1], dy[1], dz[1], mc[4], mc[5], mc[6], mc[7], mc[1], mc[2],
+                             mc[3], mc[0], ox[0], oy[0], oz[0], ox[1], oy[1], oz[1]);
             }
         }
 #pragma unroll
         for (int fi = 0; fi < NF; fi++) {
-            oxq[fi][7] = ox[fi];
-            ry[fi][xp & 3] = oy[fi];
-            rz[fi][xp & 3] = oz[fi];
-        }
-        if (x < xs) continue;  // warm-up iterations produce temporaries only
-        // ---- exchange plane-x temporaries along y and z ------------------------------
-        // (soy/soz of the previous iteration were last read before this iteration's
-        // first barrier, so they can be overwritten now)
 #pragma unroll
-        for (int fi = 0; fi < NF; fi++) {
-            soy[fi][r][l] = ry[fi][x & 3];
-            if (r >= 4 && r < 4 + FYO) soz[fi][r - 4][l] = rz[fi][x & 3];
+            for (int j = 0; j < 7; j++) oxq[fi][j] = oxq[fi][j + 1];
+            oxq[fi][7] = ox[fi];
+            S.soy[sw][fi][r * ZW + l] = oy[fi];
+            if (r >= 4 && r < 4 + YO) S.soz[sw][fi][(r - 4) * ZW + l] = oz[fi];
         }
-        __syncthreads();
-        if (!out_thread) continue;
-        const long k = x * g.nynz + colk;
-        const bool xU = x >= 4 && x < g.nx - 4;
-        if (!(xU && yzU)) {
+        bq[sw] = mc[0];
+        // ---- 4. D- and leapfrog at plane x (temporaries of x written three steps ago) ----
+        if (x < xs || x >= xe || !out_thread) return;
+        const uint32_t so = launder(ooff);  // zero extension next to the stores
+        if (!(x >= 4 && x < nx - 4 && yzU)) {
             // annulus: the spatial output is zeroed, the wavefield is not updated
             if (A.pSpace) {
-                A.pSpace[k] = 0.0f;
-                if (NF == 2) A.mSpace[k] = 0.0f;
+                gst(0.0f, A.pSpace, x, nynz, so);
+                if constexpr (NF == 2) gst(0.0f, A.mSpace, x, nynz, so);
             }
-            continue;
+            return;
         }
         float L[NF];
 #pragma unroll
         for (int fi = 0; fi < NF; fi++) {
-            if (z == 0) {
+            if (FST && z == 0) {
                 L[fi] = 0.0f + 0.0f + 0.0f;
                 continue;
             }
             float a[8];
             const float dxm = g.idx * sten8(oxq[fi]);
 #pragma unroll
-            for (int j = 0; j < 8; j++) a[j] = soy[fi][r - 4 + j][l];  // rows y-4 .. y+3
+            for (int j = 0; j < 8; j++) a[j] = S.soy[sr][fi][(r - 4 + j) * ZW + l];  // rows y-4 .. y+3
             const float dym = g.idy * sten8(a);
-            if (g.fs && z < 4) {
+            if (FST && z < 4) {
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    const long zz = z - 4 + j;           // even mirror: t[-j] = t[j-1]
-                    const long zm = zz >= 0 ? zz : -zz - 1;
-                    a[j] = soz[fi][r - 4][(int)(zm - (z0 - 4))];
+                    const int zz = z - 4 + j;  // even mirror: t[-j] = t[j-1]; lane of kz is kz+4
+                    const int zm = zz >= 0 ? zz : -zz - 1;
+                    a[j] = S.soz[sr][fi][(r - 4) * ZW + zm + 4];
                 }
             } else {
 #pragma unroll
-                for (int j = 0; j < 8; j++) a[j] = soz[fi][r - 4][l - 4 + j];  // lanes z-4 .. z+3
+                for (int j = 0; j < 8; j++) a[j] = S.soz[sr][fi][(r - 4) * ZW + l - 4 + j];  // lanes z-4 .. z+3
             }
             const float dzm = g.idz * sten8(a);
             L[fi] = dxm + dym + dzm;
         }
-        const float V = A.v[k];
-        const float dt2V2_B = A.dt2 * V * V / bq[x & 3];
-        const float qk = A.q[k];
-        const float pc = pq[0][0];
-        A.pOld[k] = leapfrog(dt2V2_B, L[0], qk, pc, A.pOld[k]);
-        if (A.pSpace) A.pSpace[k] = L[0];
+        const float V = oc[0];
+        const float dt2V2_B = A.dt2 * V * V / bq[sr];
+        const float qk = oc[1];
+        gst(leapfrog(dt2V2_B, L[0], qk, pq[0][0], oc[2]), A.pOld, x, nynz, so);
+        if (A.pSpace) gst(L[0], A.pSpace, x, nynz, so);
         if constexpr (NF == 2) {
-            const float mc = pq[1][0];
-            A.mOld[k] = leapfrog(dt2V2_B, L[1], qk, mc, A.mOld[k]);
-            if (A.mSpace) A.mSpace[k] = L[1];
+            gst(leapfrog(dt2V2_B, L[1], qk, pq[1][0], oc[3]), A.mOld, x, nynz, so);
+            if (A.mSpace) gst(L[1], A.mSpace, x, nynz, so);
         }
+    };
+
+    // steps x < xs only fill the queues; steps x >= xe (at most 3, to keep the 4-step
+    // unrolling) store nothing.  Values these steps load from planes that neighbouring chunks
+    // update (pOld, mOld) are never used.
+    for (int xb = xfirst; xb < xe; xb += 4) {
+        iter(IC<0>{}, xb);
+        iter(IC<1>{}, xb + 1);
+        iter(IC<2>{}, xb + 2);
+        iter(IC<3>{}, xb + 3);
     }
 }
 
-static inline long fused_cx(const Prop *p) {
-    if (const char *e = std::getenv("WAVEFD_FUSED_CX")) {
-        long v = std::atol(e);
-        if (v > 0) return v;
+template <int PHYS, int ZW, int NROW>
+__global__ void __launch_bounds__(ZW *NROW) k_fused_step(FusedArgs<PHYS> A) {
+    __shared__ FusedLds<PHYS == ISO ? 1 : 2, ZW, NROW> S;
+    // XCD-aware numbering: workgroup id bid runs on XCD bid mod 8; give each XCD a contiguous
+    // run of the tile order, so that the workgroups resident on one XCD at a time are
+    // neighbours and share their halo lines in its L2
+    const int nwg = (int)gridDim.x, bid = (int)blockIdx.x;
+    const int q8 = nwg >> 3, r8 = nwg & 7, xcd = bid & 7, loc = bid >> 3;
+    const int tile = (xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + loc;
+    const int per_xc = A.nzt * A.nyt;
+    const int xc = tile / per_xc, rem = tile - xc * per_xc;
+    int zt, yt;
+    if (A.bz <= 0) {  // z fastest, then y
+        zt = rem % A.nzt;
+        yt = rem / A.nzt;
+    } else {
+        // rows of by y-tiles; inside a row, blocks of bz x (rows in the row) tiles, the last
+        // row and the last block of a row ragged
+        const int yb = rem / (A.by * A.nzt), r2 = rem - yb * A.by * A.nzt;
+        const int hy = min(A.by, A.nyt - yb * A.by);
+        const int zb = r2 / (A.bz * hy), r3 = r2 - zb * A.bz * hy;
+        const int wz = min(A.bz, A.nzt - zb * A.bz);
+        zt = zb * A.bz + r3 % wz;
+        yt = yb * A.by + r3 / wz;
     }
-    // enough workgroups to fill 256 CUs several times over, without paying the 7-plane
-    // warm-up of each chunk too often
-    const long tiles = ((p->nz + FZO - 1) / FZO) * ((p->ny + FYO - 1) / FYO);
-    long cx = p->nx;
-    while (cx > 32 && tiles * ((p->nx + cx - 1) / cx) < 2048) cx = (cx + 1) / 2;
-    return cx;
+    if (A.g.fs && zt == 0)
+        fused_body<PHYS, ZW, NROW, true>(A, S, zt, yt, xc);
+    else
+        fused_body<PHYS, ZW, NROW, false>(A, S, zt, yt, xc);
+}
+
+// x-planes per chunk: balance whole rounds of workgroups (one per CU) against the 7-step
+// warm-up and the padding to a multiple of 4 steps that each chunk pays
+static inline int fused_cx(const Prop *p, long tiles) {
+    if (const char *e = std::getenv("WAVEFD_FUSED_CX")) {
+        const long v = std::atol(e);
+        if (v > 0) return (int)std::min<long>(v, p->nx);
+    }
+    const long cus = p->cu_count > 0 ? p->cu_count : 256;
+    long best_cx = p->nx;
+    double best = 1e300;
+    for (long nxc = 1; nxc <= 64 && nxc <= p->nx; nxc++) {
+        const long cx = (p->nx + nxc - 1) / nxc;
+        const long steps = (cx + 7 + 3) / 4 * 4;
+        const long rounds = (tiles * nxc + cus - 1) / cus;
+        const double cost = (double)rounds * (double)steps;
+        if (cost < best * 0.999) {
+            best = cost;
+            best_cx = cx;
+        }
+    }
+    return (int)best_cx;
+}
+
+template <int PHYS, int ZW, int NROW>
+static inline void launch_fused(Prop *p, bool linear) {
+    constexpr int ZO = ZW - 7, YO = NROW - 7;
+    const Field *f = p->f;
+    FusedArgs<PHYS> A;
+    A.g = p->g;
+    A.dt2 = p->dt * p->dt;
+    A.nzt = (int)((p->nz + ZO - 1) / ZO);
+    A.nyt = (int)((p->ny + YO - 1) / YO);
+    A.cx = fused_cx(p, (long)A.nzt * A.nyt);
+    // blocks of (CUs per XCD) tiles, 4 along z
+    const int xcd_cus = std::max(1, p->cu_count / 8);
+    A.bz = std::min(4, A.nzt);
+    A.by = std::max(1, xcd_cus / A.bz);
+    if (const char *e = std::getenv("WAVEFD_FUSED_ORDER"))
+        if (std::atoi(e) == 0) A.bz = 0;
+    const long nxc = (p->nx + A.cx - 1) / A.cx;
+    A.pCur = f[F_PCUR].d;
+    A.pOld = f[F_POLD].d;
+    A.pSpace = linear ? nullptr : f[F_PSPACE].d;
+    A.mCur = f[F_MCUR].d;
+    A.mOld = f[F_MOLD].d;
+    A.mSpace = linear ? nullptr : f[F_MSPACE].d;
+    A.v = f[F_V].d;
+    A.b = f[F_B].d;
+    A.q = f[F_Q].d;
+    A.eps = f[F_EPS].d;
+    A.eta = f[F_ETA].d;
+    A.f = f[F_F].d;
+    A.sT = f[F_ST].d;
+    A.cT = f[F_CT].d;
+    A.sP = f[F_SP].d;
+    A.cP = f[F_CP].d;
+    const long nwg = (long)A.nzt * A.nyt * nxc;
+    k_fused_step<PHYS, ZW, NROW><<<dim3((unsigned)nwg), dim3(ZW * NROW), 0, p->stream>>>(A);
 }
 
 static inline bool step_fused(Prop *p, bool linear) {
-    const Field *f = p->f;
-    const long cx = fused_cx(p);
-    dim3 grid((unsigned)((p->nz + FZO - 1) / FZO), (unsigned)((p->ny + FYO - 1) / FYO),
-              (unsigned)((p->nx + cx - 1) / cx));
-    dim3 block(FZ, FYE, 1);
-#define WFD_FILL_ARGS(A)                                                            \
-    A.g = p->g;                                                                     \
-    A.dt2 = p->dt * p->dt;                                                          \
-    A.cx = cx;                                                                      \
-    A.pCur = f[F_PCUR].d;                                                           \
-    A.pOld = f[F_POLD].d;                                                           \
-    A.pSpace = linear ? nullptr : f[F_PSPACE].d;                                    \
-    A.mCur = f[F_MCUR].d;                                                           \
-    A.mOld = f[F_MOLD].d;                                                           \
-    A.mSpace = linear ? nullptr : f[F_MSPACE].d;                                    \
-    A.v = f[F_V].d;                                                                 \
-    A.b = f[F_B].d;                                                                 \
-    A.q = f[F_Q].d;                                                                 \
-    A.eps = f[F_EPS].d;                                                             \
-    A.eta = f[F_ETA].d;                                                             \
-    A.f = f[F_F].d;                                                                 \
-    A.sT = f[F_ST].d;                                                               \
-    A.cT = f[F_CT].d;                                                               \
-    A.sP = f[F_SP].d;                                                               \
-    A.cP = f[F_CP].d;
-    if (p->phys == ISO) {
-        FusedArgs<ISO> A;
-        WFD_FILL_ARGS(A);
-        k_fused_step<ISO><<<grid, block, 0, p->stream>>>(A);
-    } else if (p->phys == VTI) {
-        FusedArgs<VTI> A;
-        WFD_FILL_ARGS(A);
-        k_fused_step<VTI><<<grid, block, 0, p->stream>>>(A);
-    } else {
-        FusedArgs<TTI> A;
-        WFD_FILL_ARGS(A);
-        k_fused_step<TTI><<<grid, block, 0, p->stream>>>(A);
+    // plane offsets are 32-bit byte offsets
+    if (p->ny * p->nz >= (1L << 29)) {
+        set_error("fused step: a y-z plane must hold fewer than 2^29 cells");
+        return false;
     }
-#undef WFD_FILL_ARGS
+    if (p->phys == ISO)
+        launch_fused<ISO, 64, 16>(p, linear);
+    else if (p->phys == VTI)
+        launch_fused<VTI, 64, 16>(p, linear);
+    else
+        launch_fused<TTI, 64, 16>(p, linear);
     WFD_LAUNCHED();
     return true;
 }

@@ -177,6 +177,7 @@ struct Prop {
     long src_nt = 0, rcv_nt = 0, src_nr = 0, rcv_nr = 0;
     int src_fields = 1, rcv_fields = 1;
     int fused = 1;  // use the fused single-pass step kernels
+    int cu_count = 256;  // compute units of the device (sizes the fused kernel's x-chunks)
 
     Mask model_mask() const {
         Mask m = bit(F_V) | bit(F_B) | bit(F_Q);
@@ -198,6 +199,11 @@ struct Prop {
             return false;
         }
         WFD_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+            cu_count = cus;
+        else
+            (void)hipGetLastError();
         const Mask m = all_mask();
         for (int i = 0; i < F_COUNT; i++) {
             if (!(m & bit(i))) continue;
