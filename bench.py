@@ -163,6 +163,52 @@ def run_gpu(physics, n, steps, warmup, seed=3):
     }
 
 
+def run_coherent(physics, n, steps, seed=3):
+    """The reference's Julia loop (test/spacetime.jl:486-492) through the default COHERENT host
+    API: the model filled through the getters' host arrays, then per step propagateforward
+    (host-visible arrays uploaded before and downloaded after), the handle swap, and
+    injectdata into the host pcur.  The rate includes every PCIe transfer that keeps host
+    arrays coherent."""
+    import torch
+    import wavefd_amd as w
+
+    dev = torch.device("cuda:0")
+    d, dt = 12.5, 1e-3
+    cls = {"iso": w.Prop3DAcoIsoDenQ_DEO2_FDTD, "vti": w.Prop3DAcoVTIDenQ_DEO2_FDTD,
+           "tti": w.Prop3DAcoTTIDenQ_DEO2_FDTD}[physics]
+    prop = cls(nz=n, ny=n, nx=n, dz=d, dy=d, dx=d, dt=dt, nsponge=40, freesurface=True, freqQ=10.0,
+               qMin=0.1, qInterior=100.0)
+    getter = {"v": w.V, "b": w.B, "eps": w.Eps, "eta": w.Eta, "f": w.F}
+    if physics == "tti":
+        getter.update({"sinTheta": w.SinTheta, "cosTheta": w.CosTheta, "sinPhi": w.SinPhi, "cosPhi": w.CosPhi})
+    model = build_model(torch, physics, n, d, dt, seed, dev)
+    for name, t in model.items():
+        # (nx,ny,nz) C-order == (nz,ny,nx) Julia order
+        getter[name](prop)[...] = t.cpu().numpy().transpose(2, 1, 0)
+    del model
+    torch.cuda.empty_cache()
+    pcur, pold = w.PCur(prop), w.POld(prop)
+    mid = (n // 2) * d
+    blocks = w.source_blocking(w.hickscoeffs(np.float32(d), np.float32(d), np.float32(d), 0.0, 0.0, 0.0,
+                                             n, n, n, [20.0], [mid + d / 2], [mid + d / 2]), 8)
+    total = steps + 1
+    wavelet = w.causal_ricker(15.0, dt * np.arange(total))[None, :].astype(np.float32)
+    w.propagateforward(prop)  # first call: host mirrors and staging allocated
+    pcur, pold = pold, pcur
+    w.injectdata(pcur, blocks, wavelet, 1)
+    t0 = time.perf_counter()
+    for it in range(2, total + 1):
+        w.propagateforward(prop)
+        pcur, pold = pold, pcur
+        w.injectdata(pcur, blocks, wavelet, it)
+    dt_wall = time.perf_counter() - t0
+    finite = bool(np.isfinite(pcur).all())
+    w.free(prop)
+    return {"physics": physics, "n": n, "steps": steps, "mcells_s": float(n) ** 3 * steps / dt_wall / 1e6,
+            "ms_per_step": dt_wall * 1e3 / steps, "finite": finite,
+            "loop": "coherent host API: propagateforward + swap + host injectdata (test/spacetime.jl:486-492)"}
+
+
 def cpu_baseline(physics, n, target_s=12.0):
     """Time the oracle's two-pass step (the reference's structure, nbx=nby=8, nbz=nz) on the
     host cores for about target_s seconds on an n^3 grid; returns Mcells/s."""
@@ -203,6 +249,22 @@ def cpu_baseline(physics, n, target_s=12.0):
                 physics.upper(), n, k, dt)}
 
 
+def measured_traffic(physics, n):
+    """HBM bytes per launch of the fused step kernel from the committed rocprofv3 --pmc runs
+    (profiles/k_fused_step_traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each
+    scaled by what the counter reads for a plain copy kernel of known size), or None when no
+    measurement of this workload is committed."""
+    path = os.path.join(ROOT, "profiles", "k_fused_step_traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh).get("%s_%d" % (physics, n))
+    except (OSError, ValueError):
+        return None
+    if not rec:
+        return None
+    return rec["fetch_kib"] * rec["fetch_scale"] * 1024 + rec["write_kib"] * rec["write_scale"] * 1024
+
+
 def copy_probe(nbytes=1 << 30, reps=3):
     """A plain copy kernel of known size: reads and writes nbytes per launch."""
     import torch
@@ -226,6 +288,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true", help="also time Iso 256^3 and VTI 512^3")
+    ap.add_argument("--coherent", action="store_true",
+                    help="also time the reference's host loop through the coherent API (PCIe-inclusive)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel 3 times (the known-size "
                          "kernel that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
@@ -244,6 +308,9 @@ def main():
         for ph, n in (("iso", 256), ("vti", 512)):
             r = run_gpu(ph, n, max(10, args.steps), args.warmup)
             extra.append({k: r[k] for k in ("physics", "n", "mcells_s", "kernel_ms", "achieved_gbs")})
+    if args.coherent:
+        for ph, n in (("iso", 256), (args.physics, min(args.n, 512))):
+            extra.append({"coherent": run_coherent(ph, n, 5)})
     cpu = None if args.no_cpu else cpu_baseline(args.physics, args.cpu_n, args.cpu_seconds)
     frac = res["achieved_gbs"] / HBM_PEAK_GBS
     line = {
@@ -272,7 +339,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(frac, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.physics, args.n),
             "bytes_per_cell": BYTES_PER_CELL[args.physics],
             "kernel_ms": round(res["kernel_ms"], 4),
         },
