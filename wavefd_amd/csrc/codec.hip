@@ -5,12 +5,13 @@
 // Format (little-endian 32-bit words):
 //   word 0      magic 0x57464443 ("WFDC")
 //   words 1..3  n1 n2 n3    (fast .. slow extents)
-//   words 4..6  b1 b2 b3    (block extents)
+//   words 4..6  b1 b2 b3    (block extents; the last block along an axis is cut to the grid)
 //   word 7      bits        (8 or 16)
 //   then one float32 scale per block (block max |x|), blocks in fast-to-slow order,
+//   then one uint32 per block: the word offset of its data from the start of the data area,
 //   then the quantised values block by block, each value round(x / scale * L) with
-//   L = 2^(bits-1) - 1, packed 4 (bits 8) or 2 (bits 16) per word, each block padded to a
-//   whole word.
+//   L = 2^(bits-1) - 1, the block's own elements only (fast to slow inside the block),
+//   packed 4 (bits 8) or 2 (bits 16) per word, each block padded to a whole word.
 // Error bound: |x - x'| <= scale_block / (2 L) per value.
 //
 // Kernels: one workgroup per block; max |x| by a workgroup reduction, then quantise and pack.
@@ -41,30 +42,45 @@ struct Geo {
     long n1, n2, n3, b1, b2, b3, m1, m2, m3;  // m: blocks per axis
     int bits;
     __host__ __device__ long nblocks() const { return m1 * m2 * m3; }
-    __host__ __device__ long bsize() const { return b1 * b2 * b3; }
-    __host__ __device__ long words_per_block() const { return bits == 8 ? (bsize() + 3) / 4 : (bsize() + 1) / 2; }
-    __host__ __device__ long total_words() const { return 8 + nblocks() + nblocks() * words_per_block(); }
+    __host__ __device__ int per() const { return bits == 8 ? 4 : 2; }
+    // extents of block blk (the last block along an axis is cut to the grid)
+    __host__ __device__ void extents(long blk, long &k1, long &k2, long &k3, long &e1, long &e2, long &e3) const {
+        k1 = blk % m1;
+        k2 = (blk / m1) % m2;
+        k3 = blk / (m1 * m2);
+        e1 = k1 == m1 - 1 ? n1 - k1 * b1 : b1;
+        e2 = k2 == m2 - 1 ? n2 - k2 * b2 : b2;
+        e3 = k3 == m3 - 1 ? n3 - k3 * b3 : b3;
+    }
+    __host__ __device__ long words_of(long blk) const {
+        long k1, k2, k3, e1, e2, e3;
+        extents(blk, k1, k2, k3, e1, e2, e3);
+        return (e1 * e2 * e3 + per() - 1) / per();
+    }
+    long data_words() const {
+        long w = 0;
+        for (long b = 0; b < nblocks(); b++) w += words_of(b);
+        return w;
+    }
+    long total_words() const { return 8 + 2 * nblocks() + data_words(); }
 };
 
 constexpr int TPB = 256;
 
-__device__ __forceinline__ bool elem(const Geo &g, long blk, long e, long &idx) {
-    const long i1 = e % g.b1, i2 = (e / g.b1) % g.b2, i3 = e / (g.b1 * g.b2);
-    const long k1 = blk % g.m1, k2 = (blk / g.m1) % g.m2, k3 = blk / (g.m1 * g.m2);
-    const long x1 = k1 * g.b1 + i1, x2 = k2 * g.b2 + i2, x3 = k3 * g.b3 + i3;
-    if (x1 >= g.n1 || x2 >= g.n2 || x3 >= g.n3) return false;
-    idx = (x3 * g.n2 + x2) * g.n1 + x1;
-    return true;
+// grid index of element e (fast to slow) of a block with origin (o1,o2,o3) and extents e1,e2
+__device__ __forceinline__ long elem_index(const Geo &g, long o1, long o2, long o3, long e1, long e2, long e) {
+    const long i1 = e % e1, i2 = (e / e1) % e2, i3 = e / (e1 * e2);
+    return ((o3 + i3) * g.n2 + (o2 + i2)) * g.n1 + (o1 + i1);
 }
 
 __global__ void __launch_bounds__(TPB) k_compress(Geo g, const float *__restrict__ x, uint32_t *__restrict__ out) {
     const long blk = blockIdx.x;
+    long k1, k2, k3, e1, e2, e3;
+    g.extents(blk, k1, k2, k3, e1, e2, e3);
+    const long o1 = k1 * g.b1, o2 = k2 * g.b2, o3 = k3 * g.b3, cnt = e1 * e2 * e3;
     __shared__ float red[TPB];
     float m = 0.0f;
-    for (long e = threadIdx.x; e < g.bsize(); e += TPB) {
-        long idx;
-        if (elem(g, blk, e, idx)) m = fmaxf(m, fabsf(x[idx]));
-    }
+    for (long e = threadIdx.x; e < cnt; e += TPB) m = fmaxf(m, fabsf(x[elem_index(g, o1, o2, o3, e1, e2, e)]));
     red[threadIdx.x] = m;
     __syncthreads();
     for (int s = TPB / 2; s > 0; s >>= 1) {
@@ -76,15 +92,16 @@ __global__ void __launch_bounds__(TPB) k_compress(Geo g, const float *__restrict
     if (threadIdx.x == 0) scales[blk] = scale;
     const int L = (1 << (g.bits - 1)) - 1;
     const float inv = scale > 0.0f ? (float)L / scale : 0.0f;
-    uint32_t *data = out + 8 + g.nblocks() + blk * g.words_per_block();
-    const int per = g.bits == 8 ? 4 : 2;
-    for (long wi = threadIdx.x; wi < g.words_per_block(); wi += TPB) {
+    const uint32_t *offs = out + 8 + g.nblocks();
+    uint32_t *data = out + 8 + 2 * g.nblocks() + offs[blk];
+    const int per = g.per();
+    const long nw = (cnt + per - 1) / per;
+    for (long wi = threadIdx.x; wi < nw; wi += TPB) {
         uint32_t word = 0;
         for (int j = 0; j < per; j++) {
             const long e = wi * per + j;
-            long idx;
             int q = 0;
-            if (e < g.bsize() && elem(g, blk, e, idx)) q = (int)rintf(x[idx] * inv);
+            if (e < cnt) q = (int)rintf(x[elem_index(g, o1, o2, o3, e1, e2, e)] * inv);
             const uint32_t u = g.bits == 8 ? (uint32_t)(q & 0xff) : (uint32_t)(q & 0xffff);
             word |= u << (j * g.bits);
         }
@@ -94,13 +111,15 @@ __global__ void __launch_bounds__(TPB) k_compress(Geo g, const float *__restrict
 
 __global__ void __launch_bounds__(TPB) k_decompress(Geo g, const uint32_t *__restrict__ in, float *__restrict__ x) {
     const long blk = blockIdx.x;
+    long k1, k2, k3, e1, e2, e3;
+    g.extents(blk, k1, k2, k3, e1, e2, e3);
+    const long o1 = k1 * g.b1, o2 = k2 * g.b2, o3 = k3 * g.b3, cnt = e1 * e2 * e3;
     const float scale = reinterpret_cast<const float *>(in + 8)[blk];
     const int L = (1 << (g.bits - 1)) - 1;
-    const uint32_t *data = in + 8 + g.nblocks() + blk * g.words_per_block();
-    const int per = g.bits == 8 ? 4 : 2;
-    for (long e = threadIdx.x; e < g.bsize(); e += TPB) {
-        long idx;
-        if (!elem(g, blk, e, idx)) continue;
+    const uint32_t *data = in + 8 + 2 * g.nblocks() + in[8 + g.nblocks() + blk];
+    const int per = g.per();
+    for (long e = threadIdx.x; e < cnt; e += TPB) {
+        const long idx = elem_index(g, o1, o2, o3, e1, e2, e);
         const uint32_t word = data[e / per];
         const int sh = (int)(e % per) * g.bits;
         int q;
@@ -162,9 +181,17 @@ long wavefd_codec_compress(const float *field, const long *n, const long *b, lon
         dout = tout;
     }
     if (e == hipSuccess) {
-        uint32_t hdr[8] = {0x57464443u, (uint32_t)g.n1, (uint32_t)g.n2, (uint32_t)g.n3,
-                           (uint32_t)g.b1, (uint32_t)g.b2, (uint32_t)g.b3, (uint32_t)g.bits};
-        e = hipMemcpy(dout, hdr, sizeof(hdr), hipMemcpyHostToDevice);
+        // header, then (after the scales the kernel writes) the block offset table
+        std::vector<uint32_t> hdr(8 + 2 * g.nblocks());
+        const uint32_t h8[8] = {0x57464443u, (uint32_t)g.n1, (uint32_t)g.n2, (uint32_t)g.n3,
+                                (uint32_t)g.b1, (uint32_t)g.b2, (uint32_t)g.b3, (uint32_t)g.bits};
+        std::memcpy(hdr.data(), h8, sizeof(h8));
+        uint32_t off = 0;
+        for (long b = 0; b < g.nblocks(); b++) {
+            hdr[8 + g.nblocks() + b] = off;
+            off += (uint32_t)g.words_of(b);
+        }
+        e = hipMemcpy(dout, hdr.data(), sizeof(uint32_t) * hdr.size(), hipMemcpyHostToDevice);
     }
     if (e == hipSuccess) {
         k_compress<<<(unsigned)g.nblocks(), TPB>>>(g, dx, dout);
@@ -197,6 +224,24 @@ long wavefd_codec_decompress(const uint32_t *in, long nwords, float *field) {
         return -1;
     }
     const long nval = g.n1 * g.n2 * g.n3, nw = g.total_words();
+    // the block offset table is fixed by the geometry: check it before any kernel follows it
+    {
+        const long nb = g.nblocks();
+        std::vector<uint32_t> offs(nb);
+        if (in_dev ? hipMemcpy(offs.data(), in + 8 + nb, sizeof(uint32_t) * nb, hipMemcpyDeviceToHost) != hipSuccess
+                   : (std::memcpy(offs.data(), in + 8 + nb, sizeof(uint32_t) * nb), false)) {
+            g_err = "codec: cannot read the block table";
+            return -1;
+        }
+        uint32_t off = 0;
+        for (long b = 0; b < nb; b++) {
+            if (offs[b] != off) {
+                g_err = "codec: corrupt block table";
+                return -1;
+            }
+            off += (uint32_t)g.words_of(b);
+        }
+    }
     const uint32_t *din = in;
     uint32_t *tin = nullptr;
     float *tx = nullptr;

@@ -43,7 +43,10 @@ struct FusedArgs {
     float dt2;
     int cx;        // x-planes per chunk
     int nzt, nyt;  // tiles along z and y
+    int nxc;       // x-chunks
     int bz, by;    // tiles per block of the XCD-local order (bz = 0: plain z-fastest order)
+    int per_xcd;     // workgroups per XCD that run at once (rounds kept in step); 0: no waiting
+    unsigned *sync;  // per-XCD count of finished workgroups, zeroed before the launch
     const float *pCur, *mCur;
     float *pOld, *mOld, *pSpace, *mSpace;
     const float *v, *b, *q, *eps, *eta, *f, *sT, *cT, *sP, *cP;
@@ -113,6 +116,7 @@ using IC = std::integral_constant<int, N>;
 template <int PHYS, int ZW, int NROW, bool FST>
 __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PHYS == ISO ? 1 : 2, ZW, NROW> &S,
                                            int zt, int yt, int xc) {
+    constexpr bool fst = FST;
     constexpr int NF = PHYS == ISO ? 1 : 2;
     constexpr int NM = PHYS == ISO ? 1 : (PHYS == VTI ? 4 : 8);  // material fields of the sandwich
     constexpr int NO = 2 + NF;                                   // v, q, pOld (, mOld)
@@ -126,12 +130,12 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PH
     const int z = z0 - 4 + l, y = y0 - 4 + r;
     const int xs = xc * A.cx;
     const int xe = xs + A.cx < nx ? xs + A.cx : nx;
-    const int zlo = FST ? 0 : 4;
+    const int zlo = fst ? 0 : 4;
     const bool yzU = y >= 4 && y < ny - 4 && z >= zlo && z < nz - 4;
     const bool out_thread = l >= 4 && l < 4 + ZO && r >= 4 && r < 4 + YO && y < ny && z < nz;
     uint32_t own_sign;
     bool own_ok;
-    const uint32_t own = col_off(y, z, ny, nz, FST, own_sign, own_ok);
+    const uint32_t own = col_off(y, z, ny, nz, fst, own_sign, own_ok);
     const uint32_t moff = yzU ? own : 0u;  // material: only used inside the update region
     const uint32_t ooff = out_thread ? own : 0u;
     // this thread's border cell of the staged plane, if any
@@ -153,7 +157,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PH
             sy = 3 + NROW + m / LZ;
             sz = m - (sy - 3 - NROW) * LZ;
         }
-        boff = col_off(y0 - 7 + sy, z0 - 7 + sz, ny, nz, FST, bsign, b_ok);
+        boff = col_off(y0 - 7 + sy, z0 - 7 + sz, ny, nz, fst, bsign, b_ok);
         blds = sy * LZ + sz;
     }
     const float *fld[2] = {A.pCur, A.mCur};
@@ -253,7 +257,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PH
                 for (int j = 0; j < 8; j++) a[j] = S.sp[s2][fi][(r + 3) * LZ + l + j];  // lanes z-3 .. z+4
                 dz[fi] = g.idz * sten8(a);
                 dx[fi] = g.idx * sten8(pq[fi]);
-                if (FST && z == 0) {  // free-surface row: x/y derivatives vanish
+                if (fst && z == 0) {  // free-surface row: x/y derivatives vanish
                     dx[fi] = 0.0f;
                     dy[fi] = 0.0f;
                 }
@@ -293,7 +297,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PH
         float L[NF];
 #pragma unroll
         for (int fi = 0; fi < NF; fi++) {
-            if (FST && z == 0) {
+            if (fst && z == 0) {
                 L[fi] = 0.0f + 0.0f + 0.0f;
                 continue;
             }
@@ -302,7 +306,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PH
 #pragma unroll
             for (int j = 0; j < 8; j++) a[j] = S.soy[sr][fi][(r - 4 + j) * ZW + l];  // rows y-4 .. y+3
             const float dym = g.idy * sten8(a);
-            if (FST && z < 4) {
+            if (fst && z < 4) {
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
                     const int zz = z - 4 + j;  // even mirror: t[-j] = t[j-1]; lane of kz is kz+4
@@ -338,18 +342,12 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PH
     }
 }
 
-template <int PHYS, int ZW, int NROW>
-__global__ void __launch_bounds__(ZW *NROW) k_fused_step(FusedArgs<PHYS> A) {
-    __shared__ FusedLds<PHYS == ISO ? 1 : 2, ZW, NROW> S;
-    // XCD-aware numbering: workgroup id bid runs on XCD bid mod 8; give each XCD a contiguous
-    // run of the tile order, so that the workgroups resident on one XCD at a time are
-    // neighbours and share their halo lines in its L2
-    const int nwg = (int)gridDim.x, bid = (int)blockIdx.x;
-    const int q8 = nwg >> 3, r8 = nwg & 7, xcd = bid & 7, loc = bid >> 3;
-    const int tile = (xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8) + loc;
+// tile index (in the XCD-local order) -> (zt, yt, xc)
+template <int PHYS>
+__device__ __forceinline__ void tile_coords(const FusedArgs<PHYS> &A, int tile, int &zt, int &yt, int &xc) {
     const int per_xc = A.nzt * A.nyt;
-    const int xc = tile / per_xc, rem = tile - xc * per_xc;
-    int zt, yt;
+    xc = tile / per_xc;
+    const int rem = tile - xc * per_xc;
     if (A.bz <= 0) {  // z fastest, then y
         zt = rem % A.nzt;
         yt = rem / A.nzt;
@@ -363,10 +361,46 @@ __global__ void __launch_bounds__(ZW *NROW) k_fused_step(FusedArgs<PHYS> A) {
         zt = zb * A.bz + r3 % wz;
         yt = yb * A.by + r3 / wz;
     }
+}
+
+template <int PHYS, int ZW, int NROW>
+__global__ void __launch_bounds__(ZW *NROW) k_fused_step(FusedArgs<PHYS> A) {
+    __shared__ FusedLds<PHYS == ISO ? 1 : 2, ZW, NROW> S;
+    // XCD-aware numbering: workgroup id bid runs on XCD bid mod 8; each XCD owns a contiguous
+    // run of the tile order, so that the workgroups resident on one XCD at a time are
+    // neighbours and share their halo lines in its L2
+    const int bid = (int)blockIdx.x, xcd = bid & 7;
+    const int ntiles = A.nzt * A.nyt * A.nxc;
+    const int q8 = ntiles >> 3, r8 = ntiles & 7;
+    const int start = xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8;
+    const int loc = bid >> 3;  // position in this XCD's run of tiles
+    int zt, yt, xc;
+    tile_coords(A, start + loc, zt, yt, xc);
+    const int per = A.per_xcd;
+    if (per > 0 && loc >= per) {
+        // Rounds in step: workgroups start on an XCD in id order, one per CU, so this
+        // workgroup belongs to round loc / per.  It waits until the XCD has finished every
+        // earlier round, so that a round's neighbouring tiles march through x together and
+        // read their shared halo lines while these are in the XCD's L2.  Those workgroups
+        // were started before this one; the wait is bounded and orders nothing the results
+        // depend on.
+        if (threadIdx.x == 0) {
+            const unsigned target = (unsigned)(loc / per * per);
+            for (int i = 0; i < 4096; i++) {
+                if (__hip_atomic_load(&A.sync[xcd], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+                __builtin_amdgcn_s_sleep(16);
+            }
+        }
+        __syncthreads();
+    }
     if (A.g.fs && zt == 0)
         fused_body<PHYS, ZW, NROW, true>(A, S, zt, yt, xc);
     else
         fused_body<PHYS, ZW, NROW, false>(A, S, zt, yt, xc);
+    if (per > 0) {
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_fetch_add(&A.sync[xcd], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // x-planes per chunk: balance whole rounds of workgroups (one per CU) against the 7-step
@@ -393,7 +427,7 @@ static inline int fused_cx(const Prop *p, long tiles) {
 }
 
 template <int PHYS, int ZW, int NROW>
-static inline void launch_fused(Prop *p, bool linear) {
+static inline bool launch_fused(Prop *p, bool linear) {
     constexpr int ZO = ZW - 7, YO = NROW - 7;
     const Field *f = p->f;
     FusedArgs<PHYS> A;
@@ -409,6 +443,7 @@ static inline void launch_fused(Prop *p, bool linear) {
     if (const char *e = std::getenv("WAVEFD_FUSED_ORDER"))
         if (std::atoi(e) == 0) A.bz = 0;
     const long nxc = (p->nx + A.cx - 1) / A.cx;
+    A.nxc = (int)nxc;
     A.pCur = f[F_PCUR].d;
     A.pOld = f[F_POLD].d;
     A.pSpace = linear ? nullptr : f[F_PSPACE].d;
@@ -425,8 +460,22 @@ static inline void launch_fused(Prop *p, bool linear) {
     A.cT = f[F_CT].d;
     A.sP = f[F_SP].d;
     A.cP = f[F_CP].d;
-    const long nwg = (long)A.nzt * A.nyt * nxc;
+    const long ntiles = (long)A.nzt * A.nyt * nxc;
+    const long nwg = ntiles;
+    A.per_xcd = 0;
+    A.sync = p->fused_sync;
+    // rounds in step when there are several rounds of one workgroup per CU (the kernel's LDS
+    // and registers allow no second one per CU).  It pays where the halo re-reads are heavy
+    // (VTI, TTI: 4 and 8 material fields; measured -20 % fetched bytes, -3 % time for TTI
+    // 800^3); for Iso the waiting costs more than the shared lines save.
+    bool lockstep = PHYS != ISO && p->cu_count % 8 == 0 && ntiles >= 2L * p->cu_count;
+    if (const char *e = std::getenv("WAVEFD_FUSED_LOCKSTEP")) lockstep = lockstep && std::atoi(e) != 0;
+    if (lockstep && p->fused_sync) {
+        A.per_xcd = p->cu_count / 8;
+        WFD_TRY(hipMemsetAsync(p->fused_sync, 0, 8 * sizeof(unsigned), p->stream));
+    }
     k_fused_step<PHYS, ZW, NROW><<<dim3((unsigned)nwg), dim3(ZW * NROW), 0, p->stream>>>(A);
+    return true;
 }
 
 static inline bool step_fused(Prop *p, bool linear) {
@@ -435,12 +484,14 @@ static inline bool step_fused(Prop *p, bool linear) {
         set_error("fused step: a y-z plane must hold fewer than 2^29 cells");
         return false;
     }
+    bool ok;
     if (p->phys == ISO)
-        launch_fused<ISO, 64, 16>(p, linear);
+        ok = launch_fused<ISO, 64, 16>(p, linear);
     else if (p->phys == VTI)
-        launch_fused<VTI, 64, 16>(p, linear);
+        ok = launch_fused<VTI, 64, 16>(p, linear);
     else
-        launch_fused<TTI, 64, 16>(p, linear);
+        ok = launch_fused<TTI, 64, 16>(p, linear);
+    if (!ok) return false;
     WFD_LAUNCHED();
     return true;
 }

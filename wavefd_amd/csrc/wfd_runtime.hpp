@@ -178,6 +178,7 @@ struct Prop {
     int src_fields = 1, rcv_fields = 1;
     int fused = 1;  // use the fused single-pass step kernels
     int cu_count = 256;  // compute units of the device (sizes the fused kernel's x-chunks)
+    unsigned *fused_sync = nullptr;  // per-XCD round counters of the persistent fused launch
 
     Mask model_mask() const {
         Mask m = bit(F_V) | bit(F_B) | bit(F_Q);
@@ -204,6 +205,7 @@ struct Prop {
             cu_count = cus;
         else
             (void)hipGetLastError();
+        WFD_TRY(hipMalloc(&fused_sync, 64));
         const Mask m = all_mask();
         for (int i = 0; i < F_COUNT; i++) {
             if (!(m & bit(i))) continue;
@@ -217,6 +219,7 @@ struct Prop {
 
     ~Prop() {
         if (stream) hipStreamSynchronize(stream);
+        if (fused_sync) (void)hipFree(fused_sync);
         for (int i = 0; i < F_COUNT; i++) {
             if (f[i].d) hipFree(f[i].d);
             if (f[i].h) hipHostFree(f[i].h);
