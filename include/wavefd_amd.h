@@ -51,6 +51,11 @@ struct SourcePoint32; /* spacetime.h */
     /* TimeStep, then inject the sources at it (1-based) into the new pCur (mCur), then extract   \
      * the receivers at it from the new pCur: the loop body of test/spacetime.jl:486-492 */       \
     void P##_TimeStepInjectExtract(void *p, long it);                                               \
+    /* the two halves alone, for loops that put other work between the step and the             \
+     * extraction (Born: step, Born source, extract; adjoint: step, adjoint source, image).       \
+     * 0 on success. */                                                                             \
+    long P##_InjectRegistered(void *p, long it);                                                    \
+    long P##_ExtractRegistered(void *p, long it);                                                   \
     /* copy the receiver traces (Julia (nt, nr)) to out (host or device).  0 on success. */       \
     long P##_GetReceiverData(void *p, float *out);                                                  \
     /* 1 (default): fused single-pass step kernel; 0: two-pass operator kernels */                 \

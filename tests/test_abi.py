@@ -19,6 +19,8 @@ HEADER_OF = {
     "vti": "prop3DAcoVTIDenQ_DEO2_FDTD.h",
     "tti": "prop3DAcoTTIDenQ_DEO2_FDTD.h",
     "spacetime": "spacetime.h",
+    "codec": "wavefd_codec.h",
+    "ginsu": "wavefd_ginsu.h",
 }
 
 REFERENCE_SYMBOLS = {
@@ -49,7 +51,7 @@ def declared(kind):
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text))
     names -= {"extern", "if", "defined"}
-    if kind != "spacetime":
+    if kind in _lib.PREFIXES:
         add = open(os.path.join(INC, "wavefd_amd.h")).read()
         add = re.sub(r"/\*.*?\*/", "", add, flags=re.S)
         for nm in re.findall(r"P##_(\w+)\s*\(", add):
@@ -62,12 +64,15 @@ def exported(path):
     return {line.split()[-1] for line in out.splitlines() if " T " in line}
 
 
-@pytest.mark.parametrize("kind", ["iso", "vti", "tti", "spacetime"])
+ALL = ["iso", "vti", "tti", "spacetime", "codec", "ginsu"]
+
+
+@pytest.mark.parametrize("kind", ALL)
 def test_library_exports_every_declared_symbol(kind):
     path = _lib.lib_path(kind)
     assert os.path.exists(path), "build first"
     names = declared(kind)
-    assert len(names) > 10
+    assert len(names) >= 4
     missing = names - exported(path)
     assert not missing, missing
 
@@ -80,13 +85,13 @@ def test_reference_symbol_set_is_covered(kind):
     assert names <= exported(_lib.lib_path(kind))
 
 
-@pytest.mark.parametrize("kind", ["iso", "vti", "tti", "spacetime"])
+@pytest.mark.parametrize("kind", ALL)
 def test_ctypes_table_matches_headers(kind):
     table = set(_lib.signatures(kind))
     assert table == declared(kind), (table ^ declared(kind))
 
 
-@pytest.mark.parametrize("kind", ["iso", "vti", "tti", "spacetime"])
+@pytest.mark.parametrize("kind", ALL)
 def test_library_loads_without_gpu(kind):
     lib = _lib.load(kind)
     assert isinstance(lib.dll, ctypes.CDLL)

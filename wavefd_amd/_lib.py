@@ -38,6 +38,8 @@ _COMMON = {
     "GetStream": (_P, [_P]),
     "SetSources": (_L, [_P, _P, _S, _P, _P, _S, _L]),
     "SetReceivers": (_L, [_P, _P, _S, _P, _S, _L]),
+    "InjectRegistered": (_L, [_P, _L]),
+    "ExtractRegistered": (_L, [_P, _L]),
     "TimeStepInjectExtract": (None, [_P, _L]),
     "GetReceiverData": (_L, [_P, _P]),
     "SetFusedStep": (None, [_P, _L]),
@@ -98,12 +100,26 @@ CODEC = {
 }
 
 
+GINSU = {
+    "wavefd_ginsu_extract": (_L, [_P, _L, _L, _L, _L, _L, _L, _P, _L, _L, _L, _P]),
+    "wavefd_ginsu_accumulate": (_L, [_P, _L, _L, _L, _L, _L, _L, _P, _L, _L, _L, _L, _L, _L, _L, _L, _L, _P]),
+    "wavefd_device_alloc": (_P, [_L]),
+    "wavefd_device_free": (None, [_P]),
+    "wavefd_device_copy": (_L, [_P, _P, _L, _P]),
+    "wavefd_device_zero": (_L, [_P, _L, _P]),
+    "wavefd_device_sync": (_L, [_P]),
+    "wavefd_ginsu_LastError": (ctypes.c_char_p, []),
+}
+
+
 def signatures(kind):
-    """{symbol: (restype, argtypes)} for 'iso' | 'vti' | 'tti' | 'spacetime' | 'codec'."""
+    """{symbol: (restype, argtypes)} for 'iso' | 'vti' | 'tti' | 'spacetime' | 'codec' | 'ginsu'."""
     if kind == "spacetime":
         return dict(SPACETIME)
     if kind == "codec":
         return dict(CODEC)
+    if kind == "ginsu":
+        return dict(GINSU)
     pre = PREFIXES[kind]
     out = {}
     for name, sig in list(_COMMON.items()) + list(_PHYS[kind].items()):
@@ -116,6 +132,8 @@ def lib_path(kind):
         name = "libspacetime.so"
     elif kind == "codec":
         name = "libwavefd_codec.so"
+    elif kind == "ginsu":
+        name = "libwavefd_ginsu.so"
     else:  # the reference's library names start in lower case: libprop3DAco...
         pre = PREFIXES[kind]
         name = "lib" + pre[0].lower() + pre[1:] + ".so"
@@ -163,6 +181,8 @@ def last_error(kind):
         msg = lib.spacetime_LastError()
     elif kind == "codec":
         msg = lib.wavefd_codec_LastError()
+    elif kind == "ginsu":
+        msg = lib.wavefd_ginsu_LastError()
     else:
         msg = lib.LastError()
     return msg.decode() if msg else ""

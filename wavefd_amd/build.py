@@ -21,6 +21,8 @@ LIBS = {
     "libspacetime.so": ("spacetime.hip", []),
     # snapshot codec behind the Compressor's lossy mode (src/compressedio.jl); additive
     "libwavefd_codec.so": ("codec.hip", []),
+    # ginsu windows, shared image and device helpers of the Born driver; additive
+    "libwavefd_ginsu.so": ("ginsu.hip", []),
 }
 
 ARCH = os.environ.get("WAVEFD_ARCH", "gfx950")

@@ -82,6 +82,12 @@
     void WFD_CAT(Prefix, TimeStepInjectExtract)(void *p, long it) {                                       \
         wfd::time_step_inject_extract(reinterpret_cast<wfd::Prop *>(p), it, false);                       \
     }                                                                                                     \
+    long WFD_CAT(Prefix, InjectRegistered)(void *p, long it) {                                            \
+        return wfd::inject_registered_call(reinterpret_cast<wfd::Prop *>(p), it) ? 0 : -1;                \
+    }                                                                                                     \
+    long WFD_CAT(Prefix, ExtractRegistered)(void *p, long it) {                                           \
+        return wfd::extract_registered_call(reinterpret_cast<wfd::Prop *>(p), it) ? 0 : -1;               \
+    }                                                                                                     \
     long WFD_CAT(Prefix, GetReceiverData)(void *p, float *out) {                                          \
         return wfd::get_receiver_data(reinterpret_cast<wfd::Prop *>(p), out) ? 0 : -1;                    \
     }                                                                                                     \

@@ -85,6 +85,22 @@ static inline bool time_step_inject_extract(Prop *p, long it, bool linear) {
     return p->end(step_writes(p, linear));
 }
 
+// the registered injection / extraction alone, between other calls (Born loops: step, Born
+// source, extract; adjoint: step, inject the adjoint source, image)
+static inline bool inject_registered_call(Prop *p, long it) {
+    const Mask m = bit(F_PCUR) | (p->phys != ISO ? bit(F_MCUR) : 0);
+    if (!p->begin(m)) return false;
+    if (!inject_registered(p, it)) return false;
+    return p->end(m);
+}
+
+static inline bool extract_registered_call(Prop *p, long it) {
+    const Mask m = bit(F_PCUR) | (p->phys != ISO ? bit(F_MCUR) : 0);
+    if (!p->begin(m)) return false;
+    if (!extract_registered(p, it)) return false;
+    return p->end(0);
+}
+
 // ---------------------------------------------------------------------------------------
 // Born source injection
 // ---------------------------------------------------------------------------------------
