@@ -65,8 +65,9 @@ def _smooth_noise(torch, shape, sigma, seed, dev):
 
 
 def build_model(torch, physics, n, d, dt, seed, dev):
-    """Return dict of (nx,ny,nz) float32 device tensors for the configured synthetic model."""
-    nx = ny = nz = n
+    """Return dict of (nx,ny,nz) float32 device tensors for the configured synthetic model
+    (n: the cube's edge, or (nx, ny, nz))."""
+    nx, ny, nz = (n, n, n) if isinstance(n, int) else n
     shape = (nx, ny, nz)
     z = torch.arange(nz, device=dev, dtype=torch.float32) * d
     vz = 1500.0 + 3000.0 * z / ((nz - 1) * d)
@@ -209,6 +210,83 @@ def run_coherent(physics, n, steps, seed=3):
             "loop": "coherent host API: propagateforward + swap + host injectdata (test/spacetime.jl:486-492)"}
 
 
+def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000)):
+    """Born shots/hour: `nshots` shots on seeded ginsu windows (per-axis sizes U{200..600}, from the
+    surface down, horizontal origin uniform in the grid) of a big (nz, ny, nx) model at 12.5 m,
+    dt 1 ms, nt steps each: background forward saving DP (and DM), Born forward with a seeded
+    +-2 % smoothed velocity perturbation, adjoint with the zero-lag imaging condition, window
+    image added into the shared image.  Source at 20 m depth in the window centre, a 16 x 16
+    receiver grid at 20 m over the window's interior.  Shots run one after another."""
+    import torch
+    from wavefd_amd import born, ginsu
+
+    dev = torch.device("cuda:0")
+    d, dt = 12.5, 1e-3
+    nz, ny, nx = big
+    rng = np.random.default_rng(seed)
+    windows = []
+    for i in range(nshots):
+        r = np.random.default_rng(seed + i)
+        sz = [int(r.integers(200, 601)) for _ in range(3)]
+        snz, sny, snx = min(sz[0], nz), min(sz[1], ny), min(sz[2], nx)
+        oy = int(r.integers(0, ny - sny + 1))
+        ox = int(r.integers(0, nx - snx + 1))
+        windows.append(ginsu.Window(big, (0, oy, ox), (snz, sny, snx)))
+    t0 = time.time()
+    hip = _hip()
+    # the big model in the bench's construction, moved into buffers of this package's runtime
+    # ((nx, ny, nz) C-order == Julia (nz, ny, nx))
+    m = build_model(torch, physics, (nx, ny, nz), d, dt, seed + 100, dev)
+    bufs = {}
+    for name in born.MODEL_FIELDS[physics]:
+        t = m[name].contiguous()
+        bufs[name] = ginsu.DeviceBuffer(t.numel() * 4)
+        torch.cuda.synchronize()
+        _copy_d2d(hip, bufs[name].ptr, t)
+    dvt = (0.02 * m["v"] * _smooth_noise(torch, (nx, ny, nz), 15, seed + 200, dev)).contiguous()  # +-2 % of v
+    torch.cuda.synchronize()
+    dvb = ginsu.DeviceBuffer(dvt.numel() * 4)
+    _copy_d2d(hip, dvb.ptr, dvt)
+    del m, dvt
+    torch.cuda.empty_cache()
+    image = ginsu.DeviceBuffer(4 * nx * ny * nz)
+    image.zero()
+    image.sync()
+    fields = {k: b.ptr for k, b in bufs.items()}
+    setup_s = time.time() - t0
+    nf = 1 if physics == "iso" else 2
+    snap = ginsu.DeviceBuffer(4 * nt * nf * max(w.ncells for w in windows))
+    wav = np.asarray(born.spacetime.causal_ricker(15.0, dt * np.arange(nt)), np.float32)
+    shots = []
+    for w in windows:
+        oz, oy, ox = w.origin
+        snz, sny, snx = w.size
+        cy, cx = (oy + sny / 2.0) * d, (ox + snx / 2.0) * d
+        ry, rx = np.meshgrid(np.linspace((oy + 30) * d, (oy + sny - 30) * d, 16),
+                             np.linspace((ox + 30) * d, (ox + snx - 30) * d, 16), indexing="ij")
+        shots.append(born.BornShot((20.0, cy, cx), np.full(ry.size, 20.0), ry.ravel(), rx.ravel(), wav))
+    t1 = time.perf_counter()
+    parts = []
+    for w, sh in zip(windows, shots):
+        out = born.born_shot(physics, w, (d, d, d), dt, sh, fields, dvb.ptr, image.ptr, nsponge=30,
+                             snapshots=snap, keep_born_data=False)
+        parts.append(out)
+    image.sync()
+    wall = time.perf_counter() - t1
+    img = image.download(np.zeros((nx, ny, nz), np.float32))
+    finite = bool(np.isfinite(img).all()) and float(np.abs(img).max()) > 0
+    del img
+    cells = sum(w.ncells for w in windows)
+    snap.free()
+    for b in list(bufs.values()) + [dvb, image]:
+        b.free()
+    return {"physics": physics, "shots": nshots, "nt": nt, "big": list(big), "wall_s": round(wall, 3),
+            "shots_per_hour": round(nshots * 3600.0 / wall, 1),
+            "mcell_steps_per_s": round(3 * nt * cells / wall / 1e6, 1), "finite": finite,
+            "setup_s": round(setup_s, 2), "window_cells_total": cells,
+            "passes": "forward (DP snapshots) + Born forward + adjoint imaging, sequential shots"}
+
+
 def cpu_baseline(physics, n, target_s=12.0):
     """Time the oracle's two-pass step (the reference's structure, nbx=nby=8, nbz=nz) on the
     host cores for about target_s seconds on an n^3 grid; returns Mcells/s."""
@@ -288,6 +366,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true", help="also time Iso 256^3 and VTI 512^3")
+    ap.add_argument("--born", choices=["iso", "vti"], default=None,
+                    help="only measure Born shots/hour on ginsu windows of a 1000x1000x600 model")
+    ap.add_argument("--born-shots", type=int, default=24)
+    ap.add_argument("--born-nt", type=int, default=200)
     ap.add_argument("--coherent", action="store_true",
                     help="also time the reference's host loop through the coherent API (PCIe-inclusive)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -296,6 +378,9 @@ def main():
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.born:
+        print(json.dumps({"born": run_born(args.born, args.born_shots, args.born_nt)}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the propagator state of one shot lives on one MI355X and this "

@@ -70,6 +70,25 @@ WAVEFD_AMD_DECLARE(Prop3DAcoTTIDenQ_DEO2_FDTD)
 
 #undef WAVEFD_AMD_DECLARE
 
+/* Born loops' per-step pairs as one fused step; no counterpart in the reference (its callers
+ * make the two calls).  Same results as the pairs, on the same arguments:
+ *   TimeStepSaveDP(p, DP[, DM])                  = TimeStep; ScaleSpatialDerivatives, with the
+ *                                                  scaled pSpace (mSpace) written to DP (DM)
+ *   TimeStepBornInjection_V(p, dV, DP[, DM])     = TimeStep; ForwardBornInjection_V(dV, DP[, DM])
+ *   TimeStepAdjointBornAccumulation_V(p, g, DP[, DM])
+ *                                                = TimeStep; AdjointBornAccumulation_V(g, DP[, DM])
+ * pSpace/mSpace are not part of the result (the fused kernel leaves them as they were).
+ * Arrays: host or device, like the reference's Born calls.  0 on success. */
+long Prop3DAcoIsoDenQ_DEO2_FDTD_TimeStepSaveDP(void *p, float *DP);
+long Prop3DAcoIsoDenQ_DEO2_FDTD_TimeStepBornInjection_V(void *p, float *dVel, float *DP);
+long Prop3DAcoIsoDenQ_DEO2_FDTD_TimeStepAdjointBornAccumulation_V(void *p, float *dVel, float *DP);
+long Prop3DAcoVTIDenQ_DEO2_FDTD_TimeStepSaveDP(void *p, float *DP, float *DM);
+long Prop3DAcoVTIDenQ_DEO2_FDTD_TimeStepBornInjection_V(void *p, float *dVel, float *DP, float *DM);
+long Prop3DAcoVTIDenQ_DEO2_FDTD_TimeStepAdjointBornAccumulation_V(void *p, float *dVel, float *DP, float *DM);
+long Prop3DAcoTTIDenQ_DEO2_FDTD_TimeStepSaveDP(void *p, float *DP, float *DM);
+long Prop3DAcoTTIDenQ_DEO2_FDTD_TimeStepBornInjection_V(void *p, float *dVel, float *DP, float *DM);
+long Prop3DAcoTTIDenQ_DEO2_FDTD_TimeStepAdjointBornAccumulation_V(void *p, float *dVel, float *DP, float *DM);
+
 #ifdef __cplusplus
 }
 #endif

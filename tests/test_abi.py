@@ -56,6 +56,8 @@ def declared(kind):
         add = re.sub(r"/\*.*?\*/", "", add, flags=re.S)
         for nm in re.findall(r"P##_(\w+)\s*\(", add):
             names.add(_lib.PREFIXES[kind] + "_" + nm)
+        for nm in re.findall(r"\b(" + _lib.PREFIXES[kind] + r"_\w+)\s*\(", add):
+            names.add(nm)
     return names
 
 

@@ -56,6 +56,9 @@ _PHYS = {
         "AdjointBornAccumulation_VB": (None, [_P, _FP, _FP, _FP, _FP]),
         "AdjointBornAccumulation_B": (None, [_P, _FP, _FP, _FP]),
         "AdjointBornAccumulation_wavefieldsep": (None, [_P, _FP, _FP, _L]),
+        "TimeStepSaveDP": (_L, [_P, _FP]),
+        "TimeStepBornInjection_V": (_L, [_P, _FP, _FP]),
+        "TimeStepAdjointBornAccumulation_V": (_L, [_P, _FP, _FP]),
         "getV": (_P, [_P]), "getB": (_P, [_P]), "getDtOmegaInvQ": (_P, [_P]),
         "getPSpace": (_P, [_P]), "getPCur": (_P, [_P]), "getPOld": (_P, [_P]),
     },
@@ -66,6 +69,9 @@ _PHYS = {
         "AdjointBornAccumulation_V": (None, [_P, _FP, _FP, _FP]),
         "AdjointBornAccumulation_wavefieldsep_V": (None, [_P, _FP, _FP, _FP, _L]),
         "AdjointBornAccumulation_VEA": (None, [_P, _FP, _FP, _FP, _FP, _FP, _FP, _FP]),
+        "TimeStepSaveDP": (_L, [_P, _FP, _FP]),
+        "TimeStepBornInjection_V": (_L, [_P, _FP, _FP, _FP]),
+        "TimeStepAdjointBornAccumulation_V": (_L, [_P, _FP, _FP, _FP]),
         "getV": (_P, [_P]), "getEps": (_P, [_P]), "getEta": (_P, [_P]), "getB": (_P, [_P]),
         "getF": (_P, [_P]), "getPSpace": (_P, [_P]), "getMSpace": (_P, [_P]), "getPCur": (_P, [_P]),
         "getPOld": (_P, [_P]), "getMCur": (_P, [_P]), "getMOld": (_P, [_P]), "getDtOmegaInvQ": (_P, [_P]),

@@ -6,7 +6,8 @@ VTI twins) and leaves the time loops to its callers (JetPackWaveFD, out of tree)
 writes those loops against this package's C-ABI, all state in HBM:
 
 1. background forward: per step ``TimeStep``, inject the source, ``ScaleSpatialDerivatives``;
-   DP_n = v^2/b L(p_{n-1}) (and DM_n) saved to a device snapshot buffer;
+   DP_n = v^2/b L(p_{n-1}) (and DM_n) saved to a device snapshot buffer (by default through the
+   fused ``TimeStepSaveDP``, and likewise the fused pairs of passes 2 and 3);
 2. Born forward (the linearisation J in dv): per step ``TimeStep``, ``ForwardBornInjection_V``
    with DP_n (p += 2 dt^2 dv/v DP_n into the new field), extract the receivers: d = J dv;
 3. adjoint (J^T): the forward step is self-adjoint in the b/(dt^2 v^2)-weighted inner product,
@@ -80,14 +81,16 @@ class BornShot:
 
 
 def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=20, freesurface=True, freqQ=5.0,
-              qMin=0.1, qInterior=100.0, snapshots=None, adjoint_data=None, keep_born_data=True):
+              qMin=0.1, qInterior=100.0, snapshots=None, adjoint_data=None, keep_born_data=True, fused=True):
     """Run the three passes of one shot on `window` and add its image into `image`.
 
     model: {'v','b'[,'eps','eta','f']} -> device address of the big grid's field; dmodel: device
     address of the big dv; image: device address of the big gradient (accumulated, +=).
     snapshots: optional DeviceBuffer of at least nt * window cells * 4 B (x2 for VTI), reused
     across shots.  adjoint_data: (nr, nt) float32 data for the adjoint pass (default: the Born
-    data of pass 2, i.e. image = J^T J dv).  Returns a dict with the Born data and timings."""
+    data of pass 2, i.e. image = J^T J dv).  fused: each pass's per-step pair as one fused step
+    (TimeStepSaveDP, TimeStepBornInjection_V, TimeStepAdjointBornAccumulation_V; the same
+    results as the separate calls).  Returns a dict with the Born data and timings."""
     if physics not in _CLS:
         raise ValueError("physics must be 'iso' or 'vti'")
     nf = 1 if physics == "iso" else 2
@@ -133,24 +136,31 @@ def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=
     prop3d.set_receivers(prop, [], nt)
     t1 = time.perf_counter()
     for it in range(1, nt + 1):
-        lib.TimeStepInjectExtract(prop.p, it)
-        lib.ScaleSpatialDerivatives(prop.p)
-        ginsu.copy(snap(0, it), prop3d.device_ptr(prop, "pSpace"), 4 * ncell, stream)
-        if nf == 2:
-            ginsu.copy(snap(1, it), prop3d.device_ptr(prop, "mSpace"), 4 * ncell, stream)
+        if fused:
+            if nf == 1:
+                lib.TimeStepSaveDP(prop.p, snap(0, it))
+            else:
+                lib.TimeStepSaveDP(prop.p, snap(0, it), snap(1, it))
+            lib.InjectRegistered(prop.p, it)
+        else:
+            lib.TimeStepInjectExtract(prop.p, it)
+            lib.ScaleSpatialDerivatives(prop.p)
+            ginsu.copy(snap(0, it), prop3d.device_ptr(prop, "pSpace"), 4 * ncell, stream)
+            if nf == 2:
+                ginsu.copy(snap(1, it), prop3d.device_ptr(prop, "mSpace"), 4 * ncell, stream)
     prop._check()
     # 2. Born forward: d = J dv
     zero_fields()
     prop3d.set_sources(prop, [], np.zeros((1, nt), F32))
     prop3d.set_receivers(prop, [rcv_pts], nt)
     t2 = time.perf_counter()
-    inj = lib.ForwardBornInjection_V
     for it in range(1, nt + 1):
-        lib.TimeStep(prop.p)
-        if nf == 1:
-            inj(prop.p, dv.ptr, snap(0, it))
+        dps = (snap(0, it),) if nf == 1 else (snap(0, it), snap(1, it))
+        if fused:
+            lib.TimeStepBornInjection_V(prop.p, dv.ptr, *dps)
         else:
-            inj(prop.p, dv.ptr, snap(0, it), snap(1, it))
+            lib.TimeStep(prop.p)
+            lib.ForwardBornInjection_V(prop.p, dv.ptr, *dps)
         lib.ExtractRegistered(prop.p, it)
     prop._check()
     born_data = prop3d.receiver_data(prop, shot.nr, nt) if (keep_born_data or adjoint_data is None) else None
@@ -166,17 +176,17 @@ def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=
     # R^T of the P-field extraction: the adjoint source goes into P only
     prop3d.set_sources(prop, [adj_pts], np.ascontiguousarray(d[:, ::-1]), fields=1)
     t3 = time.perf_counter()
-    acc = lib.AdjointBornAccumulation_V
     for m in range(1, nt + 2):
-        lib.TimeStep(prop.p)
+        n = nt + 2 - m
+        dps = () if m == 1 else ((snap(0, n),) if nf == 1 else (snap(0, n), snap(1, n)))
+        if m >= 2 and fused:  # the image reads pOld after the step: the injection below does not touch it
+            lib.TimeStepAdjointBornAccumulation_V(prop.p, gimg.ptr, *dps)
+        else:
+            lib.TimeStep(prop.p)
         if m <= nt:
             lib.InjectRegistered(prop.p, m)
-        if m >= 2:
-            n = nt + 2 - m
-            if nf == 1:
-                acc(prop.p, gimg.ptr, snap(0, n))
-            else:
-                acc(prop.p, gimg.ptr, snap(0, n), snap(1, n))
+        if m >= 2 and not fused:
+            lib.AdjointBornAccumulation_V(prop.p, gimg.ptr, *dps)
     prop._check()
     # 4. into the shared image
     ginsu.accumulate(image, window, gimg.ptr, window.interior(nsponge, freesurface), stream)

@@ -43,6 +43,19 @@ void Prop3DAcoIsoDenQ_DEO2_FDTD_AdjointBornAccumulation_wavefieldsep(void *p, fl
     wfd::wavefield_sep(reinterpret_cast<wfd::Prop *>(p), dVel, wavefieldDP, nullptr, isFWI);
 }
 
+// additive: the Born loops' per-step pairs as one fused step (include/wavefd_amd.h)
+long Prop3DAcoIsoDenQ_DEO2_FDTD_TimeStepSaveDP(void *p, float *DP) {
+    return wfd::born_step(reinterpret_cast<wfd::Prop *>(p), wfd::EPI_SAVE_DP, DP, nullptr, nullptr) ? 0 : -1;
+}
+
+long Prop3DAcoIsoDenQ_DEO2_FDTD_TimeStepBornInjection_V(void *p, float *dVel, float *DP) {
+    return wfd::born_step(reinterpret_cast<wfd::Prop *>(p), wfd::EPI_BORN_INJ, dVel, DP, nullptr) ? 0 : -1;
+}
+
+long Prop3DAcoIsoDenQ_DEO2_FDTD_TimeStepAdjointBornAccumulation_V(void *p, float *dVel, float *DP) {
+    return wfd::born_step(reinterpret_cast<wfd::Prop *>(p), wfd::EPI_IMAGE, dVel, DP, nullptr) ? 0 : -1;
+}
+
 WFD_GETTER(ISO_PREFIX, V, F_V)
 WFD_GETTER(ISO_PREFIX, B, F_B)
 WFD_GETTER(ISO_PREFIX, DtOmegaInvQ, F_Q)

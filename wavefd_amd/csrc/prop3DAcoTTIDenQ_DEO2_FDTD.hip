@@ -38,6 +38,19 @@ void Prop3DAcoTTIDenQ_DEO2_FDTD_AdjointBornAccumulation_VEA(void *p, float *dVel
                  wavefieldDM);
 }
 
+// additive: the Born loops' per-step pairs as one fused step (include/wavefd_amd.h)
+long Prop3DAcoTTIDenQ_DEO2_FDTD_TimeStepSaveDP(void *p, float *DP, float *DM) {
+    return wfd::born_step(reinterpret_cast<wfd::Prop *>(p), wfd::EPI_SAVE_DP, DP, DM, nullptr) ? 0 : -1;
+}
+
+long Prop3DAcoTTIDenQ_DEO2_FDTD_TimeStepBornInjection_V(void *p, float *dVel, float *DP, float *DM) {
+    return wfd::born_step(reinterpret_cast<wfd::Prop *>(p), wfd::EPI_BORN_INJ, dVel, DP, DM) ? 0 : -1;
+}
+
+long Prop3DAcoTTIDenQ_DEO2_FDTD_TimeStepAdjointBornAccumulation_V(void *p, float *dVel, float *DP, float *DM) {
+    return wfd::born_step(reinterpret_cast<wfd::Prop *>(p), wfd::EPI_IMAGE, dVel, DP, DM) ? 0 : -1;
+}
+
 WFD_GETTER(TTI_PREFIX, V, F_V)
 WFD_GETTER(TTI_PREFIX, Eps, F_EPS)
 WFD_GETTER(TTI_PREFIX, Eta, F_ETA)

@@ -31,6 +31,16 @@ __global__ void k_scale_space(long n, const float *__restrict__ v, const float *
     if (ms) ms[k] *= s;
 }
 
+// dp = pSpace * v^2/b (dm = mSpace * v^2/b): scaleSpatialDerivatives written to other arrays
+__global__ void k_scale_space_into(long n, const float *__restrict__ v, const float *__restrict__ b,
+                                   const float *__restrict__ ps, const float *__restrict__ ms, float *__restrict__ dp,
+                                   float *__restrict__ dm) {
+    WFD_PTW(n);
+    const float s = v[k] * v[k] / b[k];
+    dp[k] = ps[k] * s;
+    if (dm) dm[k] = ms[k] * s;
+}
+
 // out = d * in for three components (src/prop3DAcoIsoDenQ_DEO2_FDTD.h:371-373)
 __global__ void k_scale3(long n, const float *__restrict__ d, F3 in, W3 out) {
     WFD_PTW(n);

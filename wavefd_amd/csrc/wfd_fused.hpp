@@ -50,7 +50,25 @@ struct FusedArgs {
     const float *pCur, *mCur;
     float *pOld, *mOld, *pSpace, *mSpace;
     const float *v, *b, *q, *eps, *eta, *f, *sT, *cT, *sP, *cP;
+    // epilogue operands (see Epilogue)
+    float dt;
+    float *e_dp, *e_dm;             // SAVE_DP: DP, DM out; IMAGE: g (e_dp) in/out
+    const float *e_dv, *e_ip, *e_im;  // BORN_INJ: dV; BORN_INJ / IMAGE: DP, DM in
 };
+
+// Work fused after the update of each output cell (the Born loops' per-step calls, fused so
+// that their operands are read once with the step's):
+//   NONE      the reference's TimeStep: pOld <- p+, pSpace <- L (mSpace)
+//   SAVE_DP   TimeStep then ScaleSpatialDerivatives, writing DP = L v^2/b (DM) to e_dp (e_dm)
+//             instead of pSpace (src/prop3DAcoIsoDenQ_DEO2_FDTD.h:269-291)
+//   BORN_INJ  TimeStep then ForwardBornInjection_V(dV = e_dv, DP = e_ip, DM = e_im): the Born
+//             source is added to the new field (src/prop3DAcoIsoDenQ_DEO2_FDTD.h:305-334,
+//             src/prop3DAcoVTIDenQ_DEO2_FDTD.h:420-452)
+//   IMAGE     TimeStep then AdjointBornAccumulation_V(g = e_dp, DP = e_ip, DM = e_im), whose
+//             pOld after the step is this step's input field (…Iso…h:518-546, …VTI…h:564-589)
+// pSpace/mSpace are left as they were by the fused epilogues.  Every expression is the one the
+// separate kernel evaluates, in its order, annulus cells included.
+enum Epilogue { EPI_NONE = 0, EPI_SAVE_DP = 1, EPI_BORN_INJ = 2, EPI_IMAGE = 3 };
 
 template <int NF, int ZW, int NROW>
 struct FusedLds {
@@ -113,13 +131,17 @@ template <int N>
 using IC = std::integral_constant<int, N>;
 
 // FST: this tile holds the free-surface rows (fs and the first z-tile)
-template <int PHYS, int ZW, int NROW, bool FST>
+template <int PHYS, int ZW, int NROW, int EPI, bool FST>
 __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PHYS == ISO ? 1 : 2, ZW, NROW> &S,
                                            int zt, int yt, int xc) {
     constexpr bool fst = FST;
     constexpr int NF = PHYS == ISO ? 1 : 2;
     constexpr int NM = PHYS == ISO ? 1 : (PHYS == VTI ? 4 : 8);  // material fields of the sandwich
-    constexpr int NO = 2 + NF;                                   // v, q, pOld (, mOld)
+    // output operands: v, q, pOld (, mOld), then for the epilogues b and (BORN_INJ) dV, DP (, DM)
+    // or (IMAGE) g, DP (, DM)
+    constexpr int NEX = EPI == EPI_NONE ? 0 : (EPI == EPI_SAVE_DP ? 1 : 2 + NF);
+    constexpr int NO = 2 + NF + NEX;
+    constexpr int IB = 2 + NF;  // index of b among the output operands
     constexpr int ZO = ZW - 7, YO = NROW - 7, LZ = ZW + 7;
     constexpr int NB = 7 * (LZ + NROW);  // border cells of the staged plane
     static_assert(NB <= ZW * NROW, "one border cell per thread at most");
@@ -162,7 +184,17 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PH
     }
     const float *fld[2] = {A.pCur, A.mCur};
     const float *mat[8] = {A.b, A.eps, A.eta, A.f, A.sT, A.cT, A.sP, A.cP};
-    const float *oin[4] = {A.v, A.q, A.pOld, A.mOld};
+    const float *oin[NO];
+    oin[0] = A.v;
+    oin[1] = A.q;
+    oin[2] = A.pOld;
+    if constexpr (NF == 2) oin[3] = A.mOld;
+    if constexpr (EPI != EPI_NONE) oin[IB] = A.b;
+    if constexpr (EPI == EPI_BORN_INJ || EPI == EPI_IMAGE) {
+        oin[IB + 1] = EPI == EPI_BORN_INJ ? A.e_dv : A.e_dp;
+        oin[IB + 2] = A.e_ip;
+        if constexpr (NF == 2) oin[IB + 3] = A.e_im;
+    }
 
     const long nynz = g.nynz;
     // loads are issued raw; the zeroing outside the grid and the mirror's sign are applied
@@ -286,48 +318,95 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PH
         // ---- 4. D- and leapfrog at plane x (temporaries of x written three steps ago) ----
         if (x < xs || x >= xe || !out_thread) return;
         const uint32_t so = launder(ooff);  // zero extension next to the stores
-        if (!(x >= 4 && x < nx - 4 && yzU)) {
-            // annulus: the spatial output is zeroed, the wavefield is not updated
-            if (A.pSpace) {
-                gst(0.0f, A.pSpace, x, nynz, so);
-                if constexpr (NF == 2) gst(0.0f, A.mSpace, x, nynz, so);
-            }
-            return;
-        }
+        const bool upd = x >= 4 && x < nx - 4 && yzU;
         float L[NF];
 #pragma unroll
-        for (int fi = 0; fi < NF; fi++) {
-            if (fst && z == 0) {
-                L[fi] = 0.0f + 0.0f + 0.0f;
-                continue;
-            }
-            float a[8];
-            const float dxm = g.idx * sten8(oxq[fi]);
+        for (int fi = 0; fi < NF; fi++) L[fi] = 0.0f;
+        if (upd) {
 #pragma unroll
-            for (int j = 0; j < 8; j++) a[j] = S.soy[sr][fi][(r - 4 + j) * ZW + l];  // rows y-4 .. y+3
-            const float dym = g.idy * sten8(a);
-            if (fst && z < 4) {
-#pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const int zz = z - 4 + j;  // even mirror: t[-j] = t[j-1]; lane of kz is kz+4
-                    const int zm = zz >= 0 ? zz : -zz - 1;
-                    a[j] = S.soz[sr][fi][(r - 4) * ZW + zm + 4];
+            for (int fi = 0; fi < NF; fi++) {
+                if (FST && z == 0) {
+                    L[fi] = 0.0f + 0.0f + 0.0f;
+                    continue;
                 }
-            } else {
+                float a[8];
+                const float dxm = g.idx * sten8(oxq[fi]);
 #pragma unroll
-                for (int j = 0; j < 8; j++) a[j] = S.soz[sr][fi][(r - 4) * ZW + l - 4 + j];  // lanes z-4 .. z+3
+                for (int j = 0; j < 8; j++) a[j] = S.soy[sr][fi][(r - 4 + j) * ZW + l];  // rows y-4 .. y+3
+                const float dym = g.idy * sten8(a);
+                if (FST && z < 4) {
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const int zz = z - 4 + j;  // even mirror: t[-j] = t[j-1]; lane of kz is kz+4
+                        const int zm = zz >= 0 ? zz : -zz - 1;
+                        a[j] = S.soz[sr][fi][(r - 4) * ZW + zm + 4];
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; j++) a[j] = S.soz[sr][fi][(r - 4) * ZW + l - 4 + j];  // lanes z-4 .. z+3
+                }
+                const float dzm = g.idz * sten8(a);
+                L[fi] = dxm + dym + dzm;
             }
-            const float dzm = g.idz * sten8(a);
-            L[fi] = dxm + dym + dzm;
         }
         const float V = oc[0];
-        const float dt2V2_B = A.dt2 * V * V / bq[sr];
-        const float qk = oc[1];
-        gst(leapfrog(dt2V2_B, L[0], qk, pq[0][0], oc[2]), A.pOld, x, nynz, so);
-        if (A.pSpace) gst(L[0], A.pSpace, x, nynz, so);
-        if constexpr (NF == 2) {
-            gst(leapfrog(dt2V2_B, L[1], qk, pq[1][0], oc[3]), A.mOld, x, nynz, so);
-            if (A.mSpace) gst(L[1], A.mSpace, x, nynz, so);
+        // new field values (pOld's buffer): updated inside the region, kept in the annulus
+        float pn_[NF];
+        if (upd) {
+            const float dt2V2_B = A.dt2 * V * V / bq[sr];
+            const float qk = oc[1];
+            pn_[0] = leapfrog(dt2V2_B, L[0], qk, pq[0][0], oc[2]);
+            if constexpr (NF == 2) pn_[1] = leapfrog(dt2V2_B, L[1], qk, pq[1][0], oc[3]);
+        } else {
+#pragma unroll
+            for (int fi = 0; fi < NF; fi++) pn_[fi] = oc[2 + fi];
+        }
+        if constexpr (EPI == EPI_NONE) {
+            // annulus: the spatial output is zeroed, the wavefield is not updated
+            if (upd) {
+                gst(pn_[0], A.pOld, x, nynz, so);
+                if constexpr (NF == 2) gst(pn_[1], A.mOld, x, nynz, so);
+            }
+            if (A.pSpace) {
+                gst(L[0], A.pSpace, x, nynz, so);
+                if constexpr (NF == 2) gst(L[1], A.mSpace, x, nynz, so);
+            }
+        } else if constexpr (EPI == EPI_SAVE_DP) {
+            if (upd) {
+                gst(pn_[0], A.pOld, x, nynz, so);
+                if constexpr (NF == 2) gst(pn_[1], A.mOld, x, nynz, so);
+            }
+            const float sc = V * V / oc[IB];  // scaleSpatialDerivatives: pSpace *= v^2/b
+            gst(L[0] * sc, A.e_dp, x, nynz, so);
+            if constexpr (NF == 2) gst(L[1] * sc, A.e_dm, x, nynz, so);
+        } else if constexpr (EPI == EPI_BORN_INJ) {
+            const float B = oc[IB], dv = oc[IB + 1];
+            if constexpr (PHYS == ISO) {
+                const float dt2v2OverB = A.dt * A.dt * V * V / B;
+                const float factorV = 2 * B * dv / (V * V * V);
+                pn_[0] += dt2v2OverB * oc[IB + 2] * factorV;
+            } else {
+                const float factor = 2 * A.dt * A.dt * dv / V;
+                pn_[0] += factor * oc[IB + 2];
+                pn_[1] += factor * oc[IB + 3];
+            }
+            gst(pn_[0], A.pOld, x, nynz, so);
+            if constexpr (NF == 2) gst(pn_[1], A.mOld, x, nynz, so);
+        } else {  // EPI_IMAGE: g += 2b/v^3 (DP p + DM m) with this step's input fields
+            if (upd) {
+                gst(pn_[0], A.pOld, x, nynz, so);
+                if constexpr (NF == 2) gst(pn_[1], A.mOld, x, nynz, so);
+            }
+            const float B = oc[IB];
+            float gk = oc[IB + 1];
+            if constexpr (PHYS == ISO) {
+                const float factorV = +2 * B / (V * V * V);
+                gk += factorV * oc[IB + 2] * pq[0][0];
+            } else {
+                const float factor = 2 * B / (V * V * V);
+                gk += factor * (oc[IB + 2] * pq[0][0] + oc[IB + 3] * pq[1][0]);
+            }
+            gst(gk, A.e_dp, x, nynz, so);
         }
     };
 
@@ -363,7 +442,7 @@ __device__ __forceinline__ void tile_coords(const FusedArgs<PHYS> &A, int tile, 
     }
 }
 
-template <int PHYS, int ZW, int NROW>
+template <int PHYS, int ZW, int NROW, int EPI>
 __global__ void __launch_bounds__(ZW *NROW) k_fused_step(FusedArgs<PHYS> A) {
     __shared__ FusedLds<PHYS == ISO ? 1 : 2, ZW, NROW> S;
     // XCD-aware numbering: workgroup id bid runs on XCD bid mod 8; each XCD owns a contiguous
@@ -394,9 +473,9 @@ __global__ void __launch_bounds__(ZW *NROW) k_fused_step(FusedArgs<PHYS> A) {
         __syncthreads();
     }
     if (A.g.fs && zt == 0)
-        fused_body<PHYS, ZW, NROW, true>(A, S, zt, yt, xc);
+        fused_body<PHYS, ZW, NROW, EPI, true>(A, S, zt, yt, xc);
     else
-        fused_body<PHYS, ZW, NROW, false>(A, S, zt, yt, xc);
+        fused_body<PHYS, ZW, NROW, EPI, false>(A, S, zt, yt, xc);
     if (per > 0) {
         __syncthreads();
         if (threadIdx.x == 0) __hip_atomic_fetch_add(&A.sync[xcd], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -426,13 +505,31 @@ static inline int fused_cx(const Prop *p, long tiles) {
     return (int)best_cx;
 }
 
-template <int PHYS, int ZW, int NROW>
-static inline bool launch_fused(Prop *p, bool linear) {
+// e0, e1, e2: the epilogue's arrays (device), see Epilogue: SAVE_DP (DP, DM, -),
+// BORN_INJ (dV, DP, DM), IMAGE (g, DP, DM)
+template <int PHYS, int ZW, int NROW, int EPI>
+static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float *e1 = nullptr,
+                                float *e2 = nullptr) {
     constexpr int ZO = ZW - 7, YO = NROW - 7;
     const Field *f = p->f;
     FusedArgs<PHYS> A;
     A.g = p->g;
     A.dt2 = p->dt * p->dt;
+    A.dt = p->dt;
+    A.e_dp = A.e_dm = nullptr;
+    A.e_dv = A.e_ip = A.e_im = nullptr;
+    if (EPI == EPI_SAVE_DP) {
+        A.e_dp = e0;
+        A.e_dm = e1;
+    } else if (EPI == EPI_BORN_INJ) {
+        A.e_dv = e0;
+        A.e_ip = e1;
+        A.e_im = e2;
+    } else if (EPI == EPI_IMAGE) {
+        A.e_dp = e0;
+        A.e_ip = e1;
+        A.e_im = e2;
+    }
     A.nzt = (int)((p->nz + ZO - 1) / ZO);
     A.nyt = (int)((p->ny + YO - 1) / YO);
     A.cx = fused_cx(p, (long)A.nzt * A.nyt);
@@ -474,26 +571,36 @@ static inline bool launch_fused(Prop *p, bool linear) {
         A.per_xcd = p->cu_count / 8;
         WFD_TRY(hipMemsetAsync(p->fused_sync, 0, 8 * sizeof(unsigned), p->stream));
     }
-    k_fused_step<PHYS, ZW, NROW><<<dim3((unsigned)nwg), dim3(ZW * NROW), 0, p->stream>>>(A);
+    k_fused_step<PHYS, ZW, NROW, EPI><<<dim3((unsigned)nwg), dim3(ZW * NROW), 0, p->stream>>>(A);
     return true;
 }
 
-static inline bool step_fused(Prop *p, bool linear) {
+static inline bool fused_fits(Prop *p) {
     // plane offsets are 32-bit byte offsets
     if (p->ny * p->nz >= (1L << 29)) {
         set_error("fused step: a y-z plane must hold fewer than 2^29 cells");
         return false;
     }
+    return true;
+}
+
+template <int EPI>
+static inline bool step_fused_epi(Prop *p, bool linear, float *e0, float *e1, float *e2) {
+    if (!fused_fits(p)) return false;
     bool ok;
     if (p->phys == ISO)
-        ok = launch_fused<ISO, 64, 16>(p, linear);
+        ok = launch_fused<ISO, 64, 16, EPI>(p, linear, e0, e1, e2);
     else if (p->phys == VTI)
-        ok = launch_fused<VTI, 64, 16>(p, linear);
+        ok = launch_fused<VTI, 64, 16, EPI>(p, linear, e0, e1, e2);
     else
-        ok = launch_fused<TTI, 64, 16>(p, linear);
+        ok = launch_fused<TTI, 64, 16, EPI>(p, linear, e0, e1, e2);
     if (!ok) return false;
     WFD_LAUNCHED();
     return true;
+}
+
+static inline bool step_fused(Prop *p, bool linear) {
+    return step_fused_epi<EPI_NONE>(p, linear, nullptr, nullptr, nullptr);
 }
 
 }  // namespace wfd

@@ -382,4 +382,58 @@ static inline bool wavefield_sep(Prop *p, float *gV, float *DP, float *DM, long 
     return p->end(0);
 }
 
+// TimeStep with the Born loops' per-step call fused into it (additive entry points):
+//   EPI_SAVE_DP   a0 = DP, a1 = DM (out)            TimeStep + ScaleSpatialDerivatives, into DP/DM
+//   EPI_BORN_INJ  a0 = dV, a1 = DP, a2 = DM (in)    TimeStep + ForwardBornInjection_V
+//   EPI_IMAGE     a0 = g (in/out), a1 = DP, a2 = DM TimeStep + AdjointBornAccumulation_V
+// With the fused kernel switched off, the same results come from the two-pass step followed
+// by the separate kernel.  pSpace/mSpace are not part of the result.
+static inline bool born_step(Prop *p, int epi, float *a0, float *a1, float *a2) {
+    const bool two = p->phys != ISO;
+    if (!p->begin(p->all_mask())) return false;
+    Prop::Arg a[3];
+    int na = 0;
+    if (epi == EPI_SAVE_DP) {
+        WFD_ARG(a0, 0, false, true, a[na++]);
+        if (two) WFD_ARG(a1, 1, false, true, a[na++]);
+    } else {
+        WFD_ARG(a0, 0, true, epi == EPI_IMAGE, a[na++]);
+        WFD_ARG(a1, 1, true, false, a[na++]);
+        if (two) WFD_ARG(a2, 2, true, false, a[na++]);
+    }
+    float *e0 = a[0].dev, *e1 = na > 1 ? a[1].dev : nullptr, *e2 = na > 2 ? a[2].dev : nullptr;
+    if (p->fused) {
+        bool ok = epi == EPI_SAVE_DP    ? step_fused_epi<EPI_SAVE_DP>(p, false, e0, e1, nullptr)
+                  : epi == EPI_BORN_INJ ? step_fused_epi<EPI_BORN_INJ>(p, false, e0, e1, e2)
+                                        : step_fused_epi<EPI_IMAGE>(p, false, e0, e1, e2);
+        if (!ok) return false;
+        p->swap_p();
+    } else {
+        if (!step_device(p, false)) return false;
+        const Field *f = p->f;
+        if (epi == EPI_SAVE_DP) {
+            k_scale_space_into<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, f[F_V].d, f[F_B].d, f[F_PSPACE].d,
+                                                                       two ? f[F_MSPACE].d : nullptr, e0, e1);
+        } else if (epi == EPI_BORN_INJ) {
+            if (two)
+                k_vti_inj_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, f[F_V].d, e0, e1, e2, f[F_PCUR].d,
+                                                                       f[F_MCUR].d);
+            else
+                k_iso_inj_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, f[F_V].d, f[F_B].d, e0, e1,
+                                                                       f[F_PCUR].d);
+        } else {
+            if (two)
+                k_vti_acc_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, f[F_V].d, f[F_B].d, e1, e2, f[F_POLD].d,
+                                                                       f[F_MOLD].d, e0);
+            else
+                k_iso_acc_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, f[F_V].d, f[F_B].d, e1, f[F_POLD].d, e0);
+        }
+        WFD_LAUNCHED();
+    }
+    if (!p->finish_args(a, na)) return false;
+    Mask w = bit(F_PCUR) | (two ? bit(F_MCUR) : 0);
+    if (!p->fused) w |= bit(F_PSPACE) | (two ? bit(F_MSPACE) : 0);
+    return p->end(w);
+}
+
 }  // namespace wfd
