@@ -210,13 +210,17 @@ def run_coherent(physics, n, steps, seed=3):
             "loop": "coherent host API: propagateforward + swap + host injectdata (test/spacetime.jl:486-492)"}
 
 
-def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000)):
+def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000), fused=True, batch=1,
+             snapshot_gb=180.0):
     """Born shots/hour: `nshots` shots on seeded ginsu windows (per-axis sizes U{200..600}, from the
     surface down, horizontal origin uniform in the grid) of a big (nz, ny, nx) model at 12.5 m,
     dt 1 ms, nt steps each: background forward saving DP (and DM), Born forward with a seeded
     +-2 % smoothed velocity perturbation, adjoint with the zero-lag imaging condition, window
     image added into the shared image.  Source at 20 m depth in the window centre, a 16 x 16
-    receiver grid at 20 m over the window's interior.  Shots run one after another."""
+    receiver grid at 20 m over the window's interior.  fused: the per-step pairs as fused steps;
+    False: the separate calls of the reference's API.  batch > 1: up to `batch` shots (and at most
+    snapshot_gb of DP snapshots) run together, each time step of all of them one launch;
+    batch 1: shots one after another."""
     import torch
     from wavefd_amd import born, ginsu
 
@@ -255,7 +259,7 @@ def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000)):
     fields = {k: b.ptr for k, b in bufs.items()}
     setup_s = time.time() - t0
     nf = 1 if physics == "iso" else 2
-    snap = ginsu.DeviceBuffer(4 * nt * nf * max(w.ncells for w in windows))
+    snap = ginsu.DeviceBuffer(4 * nt * nf * max(w.ncells for w in windows)) if batch <= 1 else None
     wav = np.asarray(born.spacetime.causal_ricker(15.0, dt * np.arange(nt)), np.float32)
     shots = []
     for w in windows:
@@ -266,25 +270,31 @@ def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000)):
                              np.linspace((ox + 30) * d, (ox + snx - 30) * d, 16), indexing="ij")
         shots.append(born.BornShot((20.0, cy, cx), np.full(ry.size, 20.0), ry.ravel(), rx.ravel(), wav))
     t1 = time.perf_counter()
-    parts = []
-    for w, sh in zip(windows, shots):
-        out = born.born_shot(physics, w, (d, d, d), dt, sh, fields, dvb.ptr, image.ptr, nsponge=30,
-                             snapshots=snap, keep_born_data=False)
-        parts.append(out)
+    if batch > 1:
+        born.born_shots(physics, windows, (d, d, d), dt, shots, fields, dvb.ptr, image.ptr, nsponge=30, batch=batch,
+                        snapshot_bytes=snapshot_gb * 1e9, keep_born_data=False, fused=fused)
+    else:
+        for w, sh in zip(windows, shots):
+            born.born_shot(physics, w, (d, d, d), dt, sh, fields, dvb.ptr, image.ptr, nsponge=30, snapshots=snap,
+                           keep_born_data=False, fused=fused)
     image.sync()
     wall = time.perf_counter() - t1
     img = image.download(np.zeros((nx, ny, nz), np.float32))
     finite = bool(np.isfinite(img).all()) and float(np.abs(img).max()) > 0
     del img
     cells = sum(w.ncells for w in windows)
-    snap.free()
+    if snap is not None:
+        snap.free()
     for b in list(bufs.values()) + [dvb, image]:
         b.free()
     return {"physics": physics, "shots": nshots, "nt": nt, "big": list(big), "wall_s": round(wall, 3),
             "shots_per_hour": round(nshots * 3600.0 / wall, 1),
             "mcell_steps_per_s": round(3 * nt * cells / wall / 1e6, 1), "finite": finite,
             "setup_s": round(setup_s, 2), "window_cells_total": cells,
-            "passes": "forward (DP snapshots) + Born forward + adjoint imaging, sequential shots"}
+            "passes": "forward (DP snapshots) + Born forward + adjoint imaging",
+            "shots_at_once": batch if batch > 1 else 1,
+            "steps": ("one launch per time step of a batch" if batch > 1 else "fused pairs") if fused
+            else "separate calls"}
 
 
 def cpu_baseline(physics, n, target_s=12.0):
@@ -370,6 +380,8 @@ def main():
                     help="only measure Born shots/hour on ginsu windows of a 1000x1000x600 model")
     ap.add_argument("--born-shots", type=int, default=24)
     ap.add_argument("--born-nt", type=int, default=200)
+    ap.add_argument("--born-plain", action="store_true", help="Born with the separate per-step calls")
+    ap.add_argument("--born-batch", type=int, default=1, help="Born shots run together (one launch per step)")
     ap.add_argument("--coherent", action="store_true",
                     help="also time the reference's host loop through the coherent API (PCIe-inclusive)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -380,7 +392,8 @@ def main():
         copy_probe()
         return
     if args.born:
-        print(json.dumps({"born": run_born(args.born, args.born_shots, args.born_nt)}), flush=True)
+        print(json.dumps({"born": run_born(args.born, args.born_shots, args.born_nt, fused=not args.born_plain,
+                                           batch=args.born_batch)}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the propagator state of one shot lives on one MI355X and this "

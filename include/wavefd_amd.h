@@ -60,6 +60,17 @@ struct SourcePoint32; /* spacetime.h */
     long P##_GetReceiverData(void *p, float *out);                                                  \
     /* 1 (default): fused single-pass step kernel; 0: two-pass operator kernels */                 \
     void P##_SetFusedStep(void *p, long fused);                                                     \
+    /* tile shape of the fused step kernel: 0 automatic, 1 = 64 (z) x 16 (y) threads, 2 = 32 x 32;  \
+     * results do not depend on it.  0 on success, -1 for another value. */                         \
+    long P##_SetFusedTile(void *p, long tile);                                                      \
+    long P##_GetFusedTile(void *p);                                                                 \
+    /* step n (1..8) distinct handles of this physics, all in DEVICE mode, by one launch: the     \
+     * same results as calling on each handle i TimeStep (epilogue 0), TimeStepSaveDP (1: a0[i] =  \
+     * DP, a1[i] = DM), TimeStepBornInjection_V (2: a0 = dV, a1 = DP, a2 = DM) or                  \
+     * TimeStepAdjointBornAccumulation_V (3: a0 = g, a1 = DP, a2 = DM); arrays device memory,     \
+     * DM lists only for VTI/TTI.  Runs after the work queued on every handle's stream; later     \
+     * work on each handle's stream runs after it.  0 on success. */                              \
+    long P##_TimeStepBatch(void **ps, long n, long epilogue, float **a0, float **a1, float **a2);  \
     /* last error message of the library ("" if none) and reset */                                 \
     const char *P##_LastError(void);                                                                \
     void P##_ClearError(void);
@@ -82,6 +93,13 @@ WAVEFD_AMD_DECLARE(Prop3DAcoTTIDenQ_DEO2_FDTD)
 long Prop3DAcoIsoDenQ_DEO2_FDTD_TimeStepSaveDP(void *p, float *DP);
 long Prop3DAcoIsoDenQ_DEO2_FDTD_TimeStepBornInjection_V(void *p, float *dVel, float *DP);
 long Prop3DAcoIsoDenQ_DEO2_FDTD_TimeStepAdjointBornAccumulation_V(void *p, float *dVel, float *DP);
+/* Iso background and Born wavefields in one pass; no counterpart in the reference.  The same
+ * results as TimeStepSaveDP(p, DP) on the background field (pCur/pOld) followed by
+ * TimeStepBornInjection_V(q, dVel, DP) on a handle q with the same model holding the Born field,
+ * which this handle keeps as mCur/mOld (allocated and zeroed at the first call; GetDevicePtr
+ * "mCur"/"mOld" return it, registered sources with fields bit 2 inject into it and registered
+ * receivers with fields 2 extract from it).  0 on success. */
+long Prop3DAcoIsoDenQ_DEO2_FDTD_TimeStepBornCoupled_V(void *p, float *dVel, float *DP);
 long Prop3DAcoVTIDenQ_DEO2_FDTD_TimeStepSaveDP(void *p, float *DP, float *DM);
 long Prop3DAcoVTIDenQ_DEO2_FDTD_TimeStepBornInjection_V(void *p, float *dVel, float *DP, float *DM);
 long Prop3DAcoVTIDenQ_DEO2_FDTD_TimeStepAdjointBornAccumulation_V(void *p, float *dVel, float *DP, float *DM);

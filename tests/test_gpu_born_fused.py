@@ -5,7 +5,7 @@ TimeStepBornInjection_V = TimeStep; ForwardBornInjection_V       (new pCur, mCur
 TimeStepAdjointBornAccumulation_V = TimeStep; AdjointBornAccumulation_V (g, new pCur, mCur)
 
 for Iso, VTI and TTI, with and without a free surface, on a grid whose tiles are ragged, with
-the fused kernel and with the two-pass fallback (SetFusedStep 0); and the Born driver gives the
+the fused kernel's two tile shapes and with the two-pass fallback (SetFusedStep 0); and the Born driver gives the
 same Born data and image with the fused pairs as with the separate calls.
 """
 import numpy as np
@@ -30,6 +30,8 @@ def _prop(kind, fs, fused):
     for name, arr in make_fields(kind, SHAPE, 5).items():
         host_view(p, name)[:] = arr.ravel()
     p._lib.SetFusedStep(p.p, 1 if fused else 0)
+    if fused:
+        assert p._lib.SetFusedTile(p.p, fused) == 0
     return p
 
 
@@ -41,7 +43,7 @@ def _arrs(kind, seed):
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [True, False])
-@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("fused", [1, 2, 0])  # tile 64x16, tile 32x32, two-pass
 def test_save_dp(kind, fs, fused):
     a, b = _prop(kind, fs, fused), _prop(kind, fs, fused)
     a._lib.TimeStep(a.p)
@@ -57,7 +59,7 @@ def test_save_dp(kind, fs, fused):
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [True, False])
-@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("fused", [1, 2, 0])  # tile 64x16, tile 32x32, two-pass
 def test_born_injection(kind, fs, fused):
     a, b = _prop(kind, fs, fused), _prop(kind, fs, fused)
     rng = np.random.default_rng(8)
@@ -72,7 +74,7 @@ def test_born_injection(kind, fs, fused):
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [True, False])
-@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("fused", [1, 2, 0])  # tile 64x16, tile 32x32, two-pass
 def test_adjoint_accumulation(kind, fs, fused):
     a, b = _prop(kind, fs, fused), _prop(kind, fs, fused)
     rng = np.random.default_rng(10)
@@ -86,6 +88,36 @@ def test_adjoint_accumulation(kind, fs, fused):
     assert not np.array_equal(ga, g0)
     for nm in ("pCur", "pOld") + (("mCur", "mOld") if kind != "iso" else ()):
         assert np.array_equal(host_view(b, nm), host_view(a, nm)), nm
+
+
+@pytest.mark.parametrize("fs", [True, False])
+@pytest.mark.parametrize("fused", [1, 2, 0])  # tile 64x16, tile 32x32, two-pass
+def test_coupled_step(fs, fused):
+    """TimeStepBornCoupled_V on one handle = TimeStepSaveDP on a background handle followed by
+    TimeStepBornInjection_V on a handle holding the Born field, step after step"""
+    import wavefd_amd as w
+    a, b, c = _prop("iso", fs, fused), _prop("iso", fs, fused), _prop("iso", fs, fused)
+    n = int(np.prod(SHAPE))
+    for nm in ("pCur", "pOld"):  # the Born field starts at zero, like the coupled handle's
+        host_view(b, nm)[:] = 0.0
+    rng = np.random.default_rng(12)
+    dv = smooth(rng, SHAPE, 50.0, 2).ravel().copy()
+    for step in range(3):
+        dpa, dpc = np.zeros(n, F32), np.zeros(n, F32)
+        assert a._lib.TimeStepSaveDP(a.p, dpa.ctypes.data) == 0
+        assert b._lib.TimeStepBornInjection_V(b.p, dv.ctypes.data, dpa.ctypes.data) == 0
+        assert c._lib.TimeStepBornCoupled_V(c.p, dv.ctypes.data, dpc.ctypes.data) == 0
+        assert np.array_equal(dpa, dpc), step
+        for nm in ("pCur", "pOld"):
+            assert np.array_equal(host_view(c, nm), host_view(a, nm)), (step, nm)
+    from wavefd_amd import ginsu
+    for nm, dn in (("pCur", "mCur"), ("pOld", "mOld")):  # the Born field, kept as mCur/mOld
+        born = np.zeros(n, F32)
+        ptr = w.device_ptr(c, dn)
+        ginsu.copy(born.ctypes.data, ptr, born.nbytes)
+        ginsu.sync()
+        assert np.array_equal(born, host_view(b, nm)), nm
+    assert np.abs(host_view(b, "pCur")).max() > 0
 
 
 @pytest.mark.parametrize("kind", ["iso", "vti"])
@@ -107,12 +139,14 @@ def test_driver_fused_equals_separate(kind):
                          spacetime.causal_ricker(25.0, 1e-3 * np.arange(nt)))
     w = ginsu.Window((nz, ny, nx), (0, 0, 0), (nz, ny, nx))
     res = []
-    for fused in (True, False):
+    modes = [(True, None), (False, None)] + ([(True, False)] if kind == "iso" else [])
+    for fused, coupled in modes:  # Iso: coupled passes (default), separate calls, fused uncoupled
         img = ginsu.DeviceBuffer(dv.nbytes)
         img.zero()
         out = born.born_shot(kind, w, (10.0, 10.0, 10.0), 1e-3, shot, {k: q.ptr for k, q in bufs.items()}, dvb.ptr,
-                             img.ptr, nsponge=8, fused=fused)
+                             img.ptr, nsponge=8, fused=fused, coupled=coupled)
         res.append((out["born_data"], img.download(np.zeros(shape, F32))))
-    assert np.array_equal(res[0][0], res[1][0])
-    assert np.array_equal(res[0][1], res[1][1])
+    for r in res[1:]:
+        assert np.array_equal(res[0][0], r[0])
+        assert np.array_equal(res[0][1], r[1])
     assert np.abs(res[0][1]).max() > 0

@@ -55,12 +55,14 @@ def test_q_profile_matches_oracle(oracle, kind, fs):
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [False, True])
-@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("fused", ["64x16", "32x32", None])
 @pytest.mark.parametrize("shape", SHAPES)
 def test_steps_bit_exact(oracle, kind, fs, fused, shape):
     import wavefd_amd as w
     prop, orc = _pair(oracle, kind, shape, fs, 7)
-    w.set_fused_step(prop, fused)
+    w.set_fused_step(prop, fused is not None)
+    if fused:
+        w.set_fused_tile(prop, fused)
     for _ in range(5):
         w.propagateforward(prop)
         orc.time_step()

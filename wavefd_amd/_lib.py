@@ -43,6 +43,9 @@ _COMMON = {
     "TimeStepInjectExtract": (None, [_P, _L]),
     "GetReceiverData": (_L, [_P, _P]),
     "SetFusedStep": (None, [_P, _L]),
+    "SetFusedTile": (_L, [_P, _L]),
+    "GetFusedTile": (_L, [_P]),
+    "TimeStepBatch": (_L, [_P, _L, _L, _P, _P, _P]),
     "LastError": (ctypes.c_char_p, []),
     "ClearError": (None, []),
 }
@@ -58,6 +61,7 @@ _PHYS = {
         "AdjointBornAccumulation_wavefieldsep": (None, [_P, _FP, _FP, _L]),
         "TimeStepSaveDP": (_L, [_P, _FP]),
         "TimeStepBornInjection_V": (_L, [_P, _FP, _FP]),
+        "TimeStepBornCoupled_V": (_L, [_P, _FP, _FP]),
         "TimeStepAdjointBornAccumulation_V": (_L, [_P, _FP, _FP]),
         "getV": (_P, [_P]), "getB": (_P, [_P]), "getDtOmegaInvQ": (_P, [_P]),
         "getPSpace": (_P, [_P]), "getPCur": (_P, [_P]), "getPOld": (_P, [_P]),

@@ -19,9 +19,13 @@ writes those loops against this package's C-ABI, all state in HBM:
 4. the window's image is added into the shared image over the window's interior
    (``Window.interior``).
 
+For Iso, passes 1 and 2 run as one pass over two wavefields (``TimeStepBornCoupled_V``): the
+model is read once per step for both, and the Born source uses DP_n as it is formed.
+
 ``tests/test_gpu_born_pipeline.py`` checks <J dv, d> = <dv, J^T d> and J against a central
 difference of the full modelling.
 """
+import ctypes
 import time
 
 import numpy as np
@@ -80,8 +84,209 @@ class BornShot:
         return len(self.rz)
 
 
+class _ShotRun:
+    """The device state of one shot on its window: a propagator in DEVICE mode with the window's
+    model, dv, the window image, the DP (DM) snapshot area and the shot's taps."""
+
+    def __init__(self, physics, window, spacing, dt, shot, model, dmodel, nsponge, freesurface, freqQ, qMin,
+                 qInterior, snapshots):
+        self.physics, self.window, self.shot = physics, window, shot
+        self.nf = nf = 1 if physics == "iso" else 2
+        nz, ny, nx = window.size
+        dz, dy, dx = spacing
+        self.nt = nt = shot.wavelet.shape[1]
+        self.ncell = ncell = window.ncells
+        self.nsponge, self.freesurface, self.dt = nsponge, freesurface, dt
+        self.prop = prop = _CLS[physics](nz=nz, ny=ny, nx=nx, nsponge=nsponge, dz=dz, dy=dy, dx=dx, dt=dt,
+                                         freesurface=freesurface, freqQ=freqQ, qMin=qMin, qInterior=qInterior)
+        prop3d.coherence(prop, "device")
+        self.lib = prop._lib
+        self.stream = stream = prop._lib.GetStream(prop.p)
+        for name in MODEL_FIELDS[physics]:
+            ginsu.extract(model[name], window, prop3d.device_ptr(prop, name), stream)
+        self.own_snap = snapshots is None
+        if self.own_snap:
+            snapshots = ginsu.DeviceBuffer(4 * ncell * nt * nf)
+        if snapshots.nbytes < 4 * ncell * nt * nf:
+            raise ValueError("snapshot buffer too small")
+        self.snapshots = snapshots
+        self.dv = ginsu.DeviceBuffer(4 * ncell)
+        ginsu.extract(dmodel, window, self.dv.ptr, stream)
+        self.gimg = ginsu.DeviceBuffer(4 * ncell)
+        self.gimg.zero(stream)
+        # source and receiver taps in window coordinates
+        sz, sy, sx = _shift([shot.src[0]], [shot.src[1]], [shot.src[2]], window, spacing)
+        fsi = 1 if freesurface else 0  # 1-based free-surface row; 0: no image terms
+        self.src_pts = spacetime.hickscoeffs(F32(dz), F32(dy), F32(dx), 0.0, 0.0, 0.0, nz, ny, nx, sz, sy, sx,
+                                             fs_index=fsi)
+        rz, ry, rx = _shift(shot.rz, shot.ry, shot.rx, window, spacing)
+        self.rcv_pts = spacetime.linearcoeffs(F32(dz), F32(dy), F32(dx), 0.0, 0.0, 0.0, nz, ny, nx, rz, ry, rx,
+                                              fs_index=fsi)
+        self.born_data = None
+
+    def snap(self, kind, n):  # address of DP_n / DM_n, n 1-based
+        return self.snapshots.ptr + 4 * self.ncell * ((n - 1) * self.nf + kind)
+
+    def dps(self, n):
+        return (self.snap(0, n),) if self.nf == 1 else (self.snap(0, n), self.snap(1, n))
+
+    def zero_fields(self):
+        wf = ("pCur", "pOld") + (("mCur", "mOld") if self.nf == 2 else ())
+        for name in wf:
+            ginsu.zero(prop3d.device_ptr(self.prop, name), 4 * self.ncell, self.stream)
+
+    def setup_forward(self):
+        prop3d.set_sources(self.prop, [self.src_pts], self.shot.wavelet, fields=1 if self.nf == 1 else 3)
+        prop3d.set_receivers(self.prop, [], self.nt)
+
+    def setup_born(self):
+        self.zero_fields()
+        prop3d.set_sources(self.prop, [], np.zeros((1, self.nt), F32))
+        prop3d.set_receivers(self.prop, [self.rcv_pts], self.nt)
+
+    def setup_adjoint(self, adjoint_data, keep_born_data):
+        """receivers inject the time-reversed data with weights dt^2 v^2 / b (into P only: the
+        transpose of the P-field extraction)"""
+        prop, nt, dt = self.prop, self.nt, self.dt
+        if keep_born_data or adjoint_data is None:
+            self.born_data = prop3d.receiver_data(prop, self.shot.nr, nt)
+        d = self.born_data if adjoint_data is None else np.ascontiguousarray(adjoint_data, F32)
+        prop3d.synchronize(prop)  # the gather below runs on the null stream
+        vb = {k: _gather(prop3d.device_ptr(prop, k), self.rcv_pts) for k in ("v", "b")}
+        adj_pts = self.rcv_pts.copy()
+        adj_pts["c"] = (self.rcv_pts["c"].astype(np.float64) * dt * dt * vb["v"].astype(np.float64) ** 2 /
+                        vb["b"].astype(np.float64)).astype(F32)
+        self.zero_fields()
+        prop3d.set_receivers(prop, [], nt)
+        prop3d.set_sources(prop, [adj_pts], np.ascontiguousarray(d[:, ::-1]), fields=1)
+        if not keep_born_data:
+            self.born_data = None
+
+    def finish(self, image):
+        ginsu.accumulate(image, self.window, self.gimg.ptr, self.window.interior(self.nsponge, self.freesurface),
+                         self.stream)
+        prop3d.synchronize(self.prop)
+
+    def free(self):
+        self.dv.free()
+        self.gimg.free()
+        if self.own_snap:
+            self.snapshots.free()
+        prop3d.free(self.prop)
+
+
+def _batch_step(runs, epilogue, lists):
+    """one TimeStepBatch launch over the runs' propagators; lists: per-shot array addresses for
+    a0, a1, a2 (or fewer)"""
+    n = len(runs)
+    hs = (ctypes.c_void_p * n)(*[r.prop.p for r in runs])
+    arrs = [None, None, None]
+    for k, col in enumerate(lists):
+        arrs[k] = (ctypes.c_void_p * n)(*col)
+    if runs[0].lib.TimeStepBatch(hs, n, epilogue, *arrs) != 0:
+        runs[0].prop._check()
+        raise RuntimeError("TimeStepBatch failed")
+
+
+def _run_coupled(runs):
+    """passes 1 and 2 of Iso shots as one: TimeStepBornCoupled_V steps the background (saving
+    DP_n) and the Born field (with the source of that DP_n) together; the shot's source goes into
+    the background, the receivers read the Born field"""
+    nt = runs[0].nt
+    for r in runs:
+        r.setup_forward()
+        prop3d.set_receivers(r.prop, [r.rcv_pts], nt, fields=2)
+    for it in range(1, nt + 1):
+        for r in runs:
+            lib, p = r.lib, r.prop.p
+            lib.TimeStepBornCoupled_V(p, r.dv.ptr, r.snap(0, it))
+            lib.InjectRegistered(p, it)
+            lib.ExtractRegistered(p, it)
+    for r in runs:
+        r.prop._check()
+
+
+def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
+    """the three passes over a batch of shots of equal nt, in lockstep; with fused and several
+    shots, each step of all shots is one launch (TimeStepBatch); coupled (Iso, fused, one shot at
+    a time): passes 1 and 2 as one pass over both wavefields"""
+    nt = runs[0].nt
+    batched = fused and len(runs) > 1
+    if coupled and (batched or not fused or runs[0].nf != 1):
+        raise ValueError("the coupled Born pass is for Iso shots run one at a time with fused steps")
+    if coupled:
+        _run_coupled(runs)
+    # 1. background forward, saving DP (and DM)
+    for r in ([] if coupled else runs):
+        r.setup_forward()
+    for it in range(1, (0 if coupled else nt) + 1):
+        if batched:
+            _batch_step(runs, 1, list(zip(*[r.dps(it) for r in runs])))
+            for r in runs:
+                r.lib.InjectRegistered(r.prop.p, it)
+            continue
+        for r in runs:
+            lib, p = r.lib, r.prop.p
+            if fused:
+                lib.TimeStepSaveDP(p, *r.dps(it))
+                lib.InjectRegistered(p, it)
+            else:
+                lib.TimeStepInjectExtract(p, it)
+                lib.ScaleSpatialDerivatives(p)
+                ginsu.copy(r.snap(0, it), prop3d.device_ptr(r.prop, "pSpace"), 4 * r.ncell, r.stream)
+                if r.nf == 2:
+                    ginsu.copy(r.snap(1, it), prop3d.device_ptr(r.prop, "mSpace"), 4 * r.ncell, r.stream)
+    for r in runs:
+        r.prop._check()
+    # 2. Born forward: d = J dv
+    for r in ([] if coupled else runs):
+        r.setup_born()
+    for it in range(1, (0 if coupled else nt) + 1):
+        if batched:
+            _batch_step(runs, 2, [[r.dv.ptr for r in runs]] + list(zip(*[r.dps(it) for r in runs])))
+        for r in runs:
+            lib, p = r.lib, r.prop.p
+            if not batched:
+                if fused:
+                    lib.TimeStepBornInjection_V(p, r.dv.ptr, *r.dps(it))
+                else:
+                    lib.TimeStep(p)
+                    lib.ForwardBornInjection_V(p, r.dv.ptr, *r.dps(it))
+            lib.ExtractRegistered(p, it)
+    for r in runs:
+        r.prop._check()
+    # 3. adjoint with imaging.  AdjointBornAccumulation_V images pOld after the step, i.e. the
+    # step's input field: DP_{nt+2-m} after reversed step m, and one extra step images DP_1.
+    for k, r in enumerate(runs):
+        r.setup_adjoint(None if adjoint_data is None else adjoint_data[k], keep_born_data)
+    for m in range(1, nt + 2):
+        n = nt + 2 - m
+        if batched:
+            if m == 1:
+                _batch_step(runs, 0, [])
+            else:
+                _batch_step(runs, 3, [[r.gimg.ptr for r in runs]] + list(zip(*[r.dps(n) for r in runs])))
+            if m <= nt:
+                for r in runs:
+                    r.lib.InjectRegistered(r.prop.p, m)
+            continue
+        for r in runs:
+            lib, p = r.lib, r.prop.p
+            if m >= 2 and fused:  # the image reads pOld after the step: the injection below does not touch it
+                lib.TimeStepAdjointBornAccumulation_V(p, r.gimg.ptr, *r.dps(n))
+            else:
+                lib.TimeStep(p)
+            if m <= nt:
+                lib.InjectRegistered(p, m)
+            if m >= 2 and not fused:
+                lib.AdjointBornAccumulation_V(p, r.gimg.ptr, *r.dps(n))
+    for r in runs:
+        r.prop._check()
+
+
 def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=20, freesurface=True, freqQ=5.0,
-              qMin=0.1, qInterior=100.0, snapshots=None, adjoint_data=None, keep_born_data=True, fused=True):
+              qMin=0.1, qInterior=100.0, snapshots=None, adjoint_data=None, keep_born_data=True, fused=True,
+              coupled=None):
     """Run the three passes of one shot on `window` and add its image into `image`.
 
     model: {'v','b'[,'eps','eta','f']} -> device address of the big grid's field; dmodel: device
@@ -90,116 +295,73 @@ def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=
     across shots.  adjoint_data: (nr, nt) float32 data for the adjoint pass (default: the Born
     data of pass 2, i.e. image = J^T J dv).  fused: each pass's per-step pair as one fused step
     (TimeStepSaveDP, TimeStepBornInjection_V, TimeStepAdjointBornAccumulation_V; the same
-    results as the separate calls).  Returns a dict with the Born data and timings."""
+    results as the separate calls).  coupled (default: Iso with fused): passes 1 and 2 as one
+    pass over both wavefields (TimeStepBornCoupled_V; the same results again).  Returns a dict
+    with the Born data and timings."""
     if physics not in _CLS:
         raise ValueError("physics must be 'iso' or 'vti'")
-    nf = 1 if physics == "iso" else 2
-    nz, ny, nx = window.size
-    dz, dy, dx = spacing
-    nt = shot.wavelet.shape[1]
-    ncell = window.ncells
+    if coupled is None:
+        coupled = fused and physics == "iso"
     t0 = time.perf_counter()
-    prop = _CLS[physics](nz=nz, ny=ny, nx=nx, nsponge=nsponge, dz=dz, dy=dy, dx=dx, dt=dt, freesurface=freesurface,
-                         freqQ=freqQ, qMin=qMin, qInterior=qInterior)
-    prop3d.coherence(prop, "device")
-    stream = prop._lib.GetStream(prop.p)
-    for name in MODEL_FIELDS[physics]:
-        ginsu.extract(model[name], window, prop3d.device_ptr(prop, name), stream)
-    own_snap = snapshots is None
-    if own_snap:
-        snapshots = ginsu.DeviceBuffer(4 * ncell * nt * nf)
-    if snapshots.nbytes < 4 * ncell * nt * nf:
-        raise ValueError("snapshot buffer too small")
-    dv = ginsu.DeviceBuffer(4 * ncell)
-    ginsu.extract(dmodel, window, dv.ptr, stream)
-    gimg = ginsu.DeviceBuffer(4 * ncell)
-    gimg.zero(stream)
-    wf = ("pCur", "pOld") + (("mCur", "mOld") if nf == 2 else ())
-
-    def snap(kind, n):  # address of DP_n / DM_n, n 1-based
-        return snapshots.ptr + 4 * ncell * ((n - 1) * nf + kind)
-
-    def zero_fields():
-        for name in wf:
-            ginsu.zero(prop3d.device_ptr(prop, name), 4 * ncell, stream)
-
-    # receivers and source taps in window coordinates
-    sz, sy, sx = _shift([shot.src[0]], [shot.src[1]], [shot.src[2]], window, spacing)
-    fsi = 1 if freesurface else 0  # 1-based free-surface row; 0: no image terms
-    src_pts = spacetime.hickscoeffs(F32(dz), F32(dy), F32(dx), 0.0, 0.0, 0.0, nz, ny, nx, sz, sy, sx, fs_index=fsi)
-    rz, ry, rx = _shift(shot.rz, shot.ry, shot.rx, window, spacing)
-    rcv_pts = spacetime.linearcoeffs(F32(dz), F32(dy), F32(dx), 0.0, 0.0, 0.0, nz, ny, nx, rz, ry, rx,
-                                     fs_index=fsi)
-    lib = prop._lib
-    # 1. background forward, saving DP (and DM)
-    prop3d.set_sources(prop, [src_pts], shot.wavelet, fields=1 if nf == 1 else 3)
-    prop3d.set_receivers(prop, [], nt)
+    run = _ShotRun(physics, window, spacing, dt, shot, model, dmodel, nsponge, freesurface, freqQ, qMin, qInterior,
+                   snapshots)
     t1 = time.perf_counter()
-    for it in range(1, nt + 1):
-        if fused:
-            if nf == 1:
-                lib.TimeStepSaveDP(prop.p, snap(0, it))
-            else:
-                lib.TimeStepSaveDP(prop.p, snap(0, it), snap(1, it))
-            lib.InjectRegistered(prop.p, it)
-        else:
-            lib.TimeStepInjectExtract(prop.p, it)
-            lib.ScaleSpatialDerivatives(prop.p)
-            ginsu.copy(snap(0, it), prop3d.device_ptr(prop, "pSpace"), 4 * ncell, stream)
-            if nf == 2:
-                ginsu.copy(snap(1, it), prop3d.device_ptr(prop, "mSpace"), 4 * ncell, stream)
-    prop._check()
-    # 2. Born forward: d = J dv
-    zero_fields()
-    prop3d.set_sources(prop, [], np.zeros((1, nt), F32))
-    prop3d.set_receivers(prop, [rcv_pts], nt)
-    t2 = time.perf_counter()
-    for it in range(1, nt + 1):
-        dps = (snap(0, it),) if nf == 1 else (snap(0, it), snap(1, it))
-        if fused:
-            lib.TimeStepBornInjection_V(prop.p, dv.ptr, *dps)
-        else:
-            lib.TimeStep(prop.p)
-            lib.ForwardBornInjection_V(prop.p, dv.ptr, *dps)
-        lib.ExtractRegistered(prop.p, it)
-    prop._check()
-    born_data = prop3d.receiver_data(prop, shot.nr, nt) if (keep_born_data or adjoint_data is None) else None
-    # 3. adjoint with imaging: receivers inject with weights dt^2 v^2 / b, time reversed
-    d = born_data if adjoint_data is None else np.ascontiguousarray(adjoint_data, F32)
-    prop3d.synchronize(prop)  # the gather below runs on the null stream
-    vb = {k: _gather(prop3d.device_ptr(prop, k), rcv_pts) for k in ("v", "b")}
-    adj_pts = rcv_pts.copy()
-    adj_pts["c"] = (rcv_pts["c"].astype(np.float64) * dt * dt * vb["v"].astype(np.float64) ** 2 /
-                    vb["b"].astype(np.float64)).astype(F32)
-    zero_fields()
-    prop3d.set_receivers(prop, [], nt)
-    # R^T of the P-field extraction: the adjoint source goes into P only
-    prop3d.set_sources(prop, [adj_pts], np.ascontiguousarray(d[:, ::-1]), fields=1)
-    t3 = time.perf_counter()
-    for m in range(1, nt + 2):
-        n = nt + 2 - m
-        dps = () if m == 1 else ((snap(0, n),) if nf == 1 else (snap(0, n), snap(1, n)))
-        if m >= 2 and fused:  # the image reads pOld after the step: the injection below does not touch it
-            lib.TimeStepAdjointBornAccumulation_V(prop.p, gimg.ptr, *dps)
-        else:
-            lib.TimeStep(prop.p)
-        if m <= nt:
-            lib.InjectRegistered(prop.p, m)
-        if m >= 2 and not fused:
-            lib.AdjointBornAccumulation_V(prop.p, gimg.ptr, *dps)
-    prop._check()
-    # 4. into the shared image
-    ginsu.accumulate(image, window, gimg.ptr, window.interior(nsponge, freesurface), stream)
-    prop3d.synchronize(prop)
-    t4 = time.perf_counter()
-    out = {"born_data": born_data, "window": window, "t_setup": t1 - t0, "t_forward": t2 - t1, "t_born": t3 - t2,
-           "t_adjoint": t4 - t3, "t_total": t4 - t0}
-    dv.free()
-    gimg.free()
-    if own_snap:
-        snapshots.free()
-    prop3d.free(prop)
-    return out
+    try:
+        _run_passes([run], fused, None if adjoint_data is None else [adjoint_data], keep_born_data, coupled)
+        run.finish(image)
+        t2 = time.perf_counter()
+        return {"born_data": run.born_data, "window": window, "t_setup": t1 - t0, "t_passes": t2 - t1,
+                "t_total": t2 - t0}
+    finally:
+        run.free()
+
+
+def born_shots(physics, windows, spacing, dt, shots, model, dmodel, image, nsponge=20, freesurface=True, freqQ=5.0,
+               qMin=0.1, qInterior=100.0, batch=4, snapshot_bytes=None, adjoint_data=None, keep_born_data=True,
+               fused=True):
+    """Born modelling and imaging of many shots (one window each, equal nt), several at a time:
+    the shots of a batch run their passes together, each time step of all of them one launch
+    of the fused kernel (TimeStepBatch).  Batches are formed in order, at most `batch` shots and,
+    if snapshot_bytes is given, at most that many bytes of DP (DM) snapshots (a shot larger than
+    that runs alone).  Every shot's results equal born_shot's for it; the images are added into
+    `image` one shot after the other, in shot order.  Returns the list of per-shot Born data (None
+    entries unless keep_born_data)."""
+    if physics not in _CLS:
+        raise ValueError("physics must be 'iso' or 'vti'")
+    if len(windows) != len(shots):
+        raise ValueError("one window per shot")
+    nf = 1 if physics == "iso" else 2
+    nts = {s.wavelet.shape[1] for s in shots}
+    if len(nts) > 1:
+        raise ValueError("shots of a batch run must share nt")
+    nt = nts.pop() if shots else 0
+    batch = max(1, min(int(batch), 8))
+    groups, cur, cur_bytes = [], [], 0
+    for i, w in enumerate(windows):
+        b = 4 * w.ncells * nt * nf
+        if cur and (len(cur) == batch or (snapshot_bytes is not None and cur_bytes + b > snapshot_bytes)):
+            groups.append(cur)
+            cur, cur_bytes = [], 0
+        cur.append(i)
+        cur_bytes += b
+    if cur:
+        groups.append(cur)
+    data = [None] * len(shots)
+    for g in groups:
+        runs = []
+        try:
+            for i in g:
+                runs.append(_ShotRun(physics, windows[i], spacing, dt, shots[i], model, dmodel, nsponge, freesurface,
+                                     freqQ, qMin, qInterior, None))
+            _run_passes(runs, fused, None if adjoint_data is None else [adjoint_data[i] for i in g], keep_born_data,
+                        coupled=fused and physics == "iso" and len(runs) == 1)
+            for i, r in zip(g, runs):
+                r.finish(image)
+                data[i] = r.born_data
+        finally:
+            for r in runs:
+                r.free()
+    return data
 
 
 def model_shot(physics, window, spacing, dt, shot, model, nsponge=20, freesurface=True, freqQ=5.0, qMin=0.1,

@@ -56,6 +56,10 @@ long Prop3DAcoIsoDenQ_DEO2_FDTD_TimeStepAdjointBornAccumulation_V(void *p, float
     return wfd::born_step(reinterpret_cast<wfd::Prop *>(p), wfd::EPI_IMAGE, dVel, DP, nullptr) ? 0 : -1;
 }
 
+long Prop3DAcoIsoDenQ_DEO2_FDTD_TimeStepBornCoupled_V(void *p, float *dVel, float *DP) {
+    return wfd::born_coupled(reinterpret_cast<wfd::Prop *>(p), dVel, DP) ? 0 : -1;
+}
+
 WFD_GETTER(ISO_PREFIX, V, F_V)
 WFD_GETTER(ISO_PREFIX, B, F_B)
 WFD_GETTER(ISO_PREFIX, DtOmegaInvQ, F_Q)

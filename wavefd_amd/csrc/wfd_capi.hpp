@@ -92,5 +92,12 @@
         return wfd::get_receiver_data(reinterpret_cast<wfd::Prop *>(p), out) ? 0 : -1;                    \
     }                                                                                                     \
     void WFD_CAT(Prefix, SetFusedStep)(void *p, long fused) { reinterpret_cast<wfd::Prop *>(p)->fused = (int)fused; } \
+    long WFD_CAT(Prefix, SetFusedTile)(void *p, long tile) {                                              \
+        return wfd::set_fused_tile(reinterpret_cast<wfd::Prop *>(p), tile) ? 0 : -1;                      \
+    }                                                                                                     \
+    long WFD_CAT(Prefix, GetFusedTile)(void *p) { return reinterpret_cast<wfd::Prop *>(p)->fused_tile; }  \
+    long WFD_CAT(Prefix, TimeStepBatch)(void **ps, long n, long epilogue, float **a0, float **a1, float **a2) { \
+        return wfd::time_step_batch(PHYS, ps, n, epilogue, a0, a1, a2) ? 0 : -1;                          \
+    }                                                                                                     \
     const char *WFD_CAT(Prefix, LastError)(void) { return wfd::g_last_error.c_str(); }                    \
     void WFD_CAT(Prefix, ClearError)(void) { wfd::g_last_error.clear(); }

@@ -66,9 +66,15 @@ struct FusedArgs {
 //             src/prop3DAcoVTIDenQ_DEO2_FDTD.h:420-452)
 //   IMAGE     TimeStep then AdjointBornAccumulation_V(g = e_dp, DP = e_ip, DM = e_im), whose
 //             pOld after the step is this step's input field (…Iso…h:518-546, …VTI…h:564-589)
+//   BORN_COUPLED (Iso only) two wavefields in one pass: the background (pCur/pOld) takes
+//             SAVE_DP's step, writing DP to e_dp, and the Born field (mCur/mOld) takes
+//             BORN_INJ's step with dV = e_dv and that same DP
 // pSpace/mSpace are left as they were by the fused epilogues.  Every expression is the one the
 // separate kernel evaluates, in its order, annulus cells included.
-enum Epilogue { EPI_NONE = 0, EPI_SAVE_DP = 1, EPI_BORN_INJ = 2, EPI_IMAGE = 3 };
+enum Epilogue { EPI_NONE = 0, EPI_SAVE_DP = 1, EPI_BORN_INJ = 2, EPI_IMAGE = 3, EPI_BORN_COUPLED = 4 };
+
+// wavefields a launch steps: p (Iso), p and m (VTI, TTI), background and Born field (coupled)
+constexpr int fused_nf(int phys, int epi) { return phys == ISO && epi != EPI_BORN_COUPLED ? 1 : 2; }
 
 template <int NF, int ZW, int NROW>
 struct FusedLds {
@@ -132,14 +138,15 @@ using IC = std::integral_constant<int, N>;
 
 // FST: this tile holds the free-surface rows (fs and the first z-tile)
 template <int PHYS, int ZW, int NROW, int EPI, bool FST>
-__device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PHYS == ISO ? 1 : 2, ZW, NROW> &S,
+__device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<fused_nf(PHYS, EPI), ZW, NROW> &S,
                                            int zt, int yt, int xc) {
     constexpr bool fst = FST;
-    constexpr int NF = PHYS == ISO ? 1 : 2;
+    constexpr int NF = fused_nf(PHYS, EPI);
+    static_assert(EPI != EPI_BORN_COUPLED || PHYS == ISO, "the coupled Born step is Iso's");
     constexpr int NM = PHYS == ISO ? 1 : (PHYS == VTI ? 4 : 8);  // material fields of the sandwich
     // output operands: v, q, pOld (, mOld), then for the epilogues b and (BORN_INJ) dV, DP (, DM)
     // or (IMAGE) g, DP (, DM)
-    constexpr int NEX = EPI == EPI_NONE ? 0 : (EPI == EPI_SAVE_DP ? 1 : 2 + NF);
+    constexpr int NEX = EPI == EPI_NONE ? 0 : (EPI == EPI_SAVE_DP ? 1 : (EPI == EPI_BORN_COUPLED ? 2 : 2 + NF));
     constexpr int NO = 2 + NF + NEX;
     constexpr int IB = 2 + NF;  // index of b among the output operands
     constexpr int ZO = ZW - 7, YO = NROW - 7, LZ = ZW + 7;
@@ -190,6 +197,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PH
     oin[2] = A.pOld;
     if constexpr (NF == 2) oin[3] = A.mOld;
     if constexpr (EPI != EPI_NONE) oin[IB] = A.b;
+    if constexpr (EPI == EPI_BORN_COUPLED) oin[IB + 1] = A.e_dv;
     if constexpr (EPI == EPI_BORN_INJ || EPI == EPI_IMAGE) {
         oin[IB + 1] = EPI == EPI_BORN_INJ ? A.e_dv : A.e_dp;
         oin[IB + 2] = A.e_ip;
@@ -295,9 +303,12 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PH
                 }
             }
             if constexpr (PHYS == ISO) {
-                ox[0] = mc[0] * dx[0];
-                oy[0] = mc[0] * dy[0];
-                oz[0] = mc[0] * dz[0];
+#pragma unroll
+                for (int fi = 0; fi < NF; fi++) {
+                    ox[fi] = mc[0] * dx[fi];
+                    oy[fi] = mc[0] * dy[fi];
+                    oz[fi] = mc[0] * dz[fi];
+                }
             } else if constexpr (PHYS == VTI) {
                 vti_material(dx[0], dy[0], dz[0], dx[1], dy[1], dz[1], mc[1], mc[2], mc[3], mc[0], ox[0], oy[0],
                              oz[0], ox[1], oy[1], oz[1]);
@@ -392,6 +403,17 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<PH
             }
             gst(pn_[0], A.pOld, x, nynz, so);
             if constexpr (NF == 2) gst(pn_[1], A.mOld, x, nynz, so);
+        } else if constexpr (EPI == EPI_BORN_COUPLED) {
+            // background: SAVE_DP's step; Born field: BORN_INJ's step with the DP just formed
+            if (upd) gst(pn_[0], A.pOld, x, nynz, so);
+            const float B = oc[IB], dv = oc[IB + 1];
+            const float sc = V * V / B;
+            const float DP = L[0] * sc;
+            gst(DP, A.e_dp, x, nynz, so);
+            const float dt2v2OverB = A.dt * A.dt * V * V / B;
+            const float factorV = 2 * B * dv / (V * V * V);
+            pn_[1] += dt2v2OverB * DP * factorV;
+            gst(pn_[1], A.mOld, x, nynz, so);
         } else {  // EPI_IMAGE: g += 2b/v^3 (DP p + DM m) with this step's input fields
             if (upd) {
                 gst(pn_[0], A.pOld, x, nynz, so);
@@ -442,19 +464,31 @@ __device__ __forceinline__ void tile_coords(const FusedArgs<PHYS> &A, int tile, 
     }
 }
 
-template <int PHYS, int ZW, int NROW, int EPI>
-__global__ void __launch_bounds__(ZW *NROW) k_fused_step(FusedArgs<PHYS> A) {
-    __shared__ FusedLds<PHYS == ISO ? 1 : 2, ZW, NROW> S;
-    // XCD-aware numbering: workgroup id bid runs on XCD bid mod 8; each XCD owns a contiguous
-    // run of the tile order, so that the workgroups resident on one XCD at a time are
-    // neighbours and share their halo lines in its L2
-    const int bid = (int)blockIdx.x, xcd = bid & 7;
+// workgroup bid of a launch of nwg workgroups over A's tiles -> (zt, yt, xc); false for the
+// padding workgroups of a batch (nwg rounded up to a multiple of 8).  XCD-aware numbering:
+// workgroup id bid runs on XCD bid mod 8; each XCD owns a contiguous run of the tile order, so
+// that the workgroups resident on one XCD at a time are neighbours and share their halo lines
+// in its L2.
+template <int PHYS>
+__device__ __forceinline__ bool fused_tile_of_wg(const FusedArgs<PHYS> &A, int bid, int &loc, int &zt, int &yt,
+                                                 int &xc) {
+    const int xcd = bid & 7;
     const int ntiles = A.nzt * A.nyt * A.nxc;
     const int q8 = ntiles >> 3, r8 = ntiles & 7;
     const int start = xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8;
-    const int loc = bid >> 3;  // position in this XCD's run of tiles
-    int zt, yt, xc;
+    const int count = q8 + (xcd < r8 ? 1 : 0);
+    loc = bid >> 3;  // position in this XCD's run of tiles
+    if (loc >= count) return false;
     tile_coords(A, start + loc, zt, yt, xc);
+    return true;
+}
+
+template <int PHYS, int ZW, int NROW, int EPI>
+__global__ void __launch_bounds__(ZW *NROW) k_fused_step(FusedArgs<PHYS> A) {
+    __shared__ FusedLds<fused_nf(PHYS, EPI), ZW, NROW> S;
+    const int bid = (int)blockIdx.x, xcd = bid & 7;
+    int loc, zt, yt, xc;
+    if (!fused_tile_of_wg(A, bid, loc, zt, yt, xc)) return;
     const int per = A.per_xcd;
     if (per > 0 && loc >= per) {
         // Rounds in step: workgroups start on an XCD in id order, one per CU, so this
@@ -482,6 +516,32 @@ __global__ void __launch_bounds__(ZW *NROW) k_fused_step(FusedArgs<PHYS> A) {
     }
 }
 
+// Several grids (shots) stepped by one launch: grid s owns workgroups [wg[s], wg[s+1]), each
+// range a multiple of 8 long so that the XCD-aware numbering holds inside every range.  The
+// grids share the physics, the tile shape and the epilogue; sizes and arrays are their own.
+constexpr int FUSED_MAX_BATCH = 8;
+template <int PHYS>
+struct FusedBatch {
+    FusedArgs<PHYS> a[FUSED_MAX_BATCH];
+    int wg[FUSED_MAX_BATCH + 1];
+    int n;
+};
+
+template <int PHYS, int ZW, int NROW, int EPI>
+__global__ void __launch_bounds__(ZW *NROW) k_fused_batch(FusedBatch<PHYS> B) {
+    __shared__ FusedLds<fused_nf(PHYS, EPI), ZW, NROW> S;
+    const int bid = (int)blockIdx.x;
+    int s = 0;
+    while (s + 1 < B.n && bid >= B.wg[s + 1]) s++;
+    const FusedArgs<PHYS> &A = B.a[s];
+    int loc, zt, yt, xc;
+    if (!fused_tile_of_wg(A, bid - B.wg[s], loc, zt, yt, xc)) return;
+    if (A.g.fs && zt == 0)
+        fused_body<PHYS, ZW, NROW, EPI, true>(A, S, zt, yt, xc);
+    else
+        fused_body<PHYS, ZW, NROW, EPI, false>(A, S, zt, yt, xc);
+}
+
 // x-planes per chunk: balance whole rounds of workgroups (one per CU) against the 7-step
 // warm-up and the padding to a multiple of 4 steps that each chunk pays
 static inline int fused_cx(const Prop *p, long tiles) {
@@ -506,13 +566,11 @@ static inline int fused_cx(const Prop *p, long tiles) {
 }
 
 // e0, e1, e2: the epilogue's arrays (device), see Epilogue: SAVE_DP (DP, DM, -),
-// BORN_INJ (dV, DP, DM), IMAGE (g, DP, DM)
+// BORN_INJ (dV, DP, DM), IMAGE (g, DP, DM).  Returns the number of tiles.
 template <int PHYS, int ZW, int NROW, int EPI>
-static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float *e1 = nullptr,
-                                float *e2 = nullptr) {
+static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float *e2, FusedArgs<PHYS> &A) {
     constexpr int ZO = ZW - 7, YO = NROW - 7;
     const Field *f = p->f;
-    FusedArgs<PHYS> A;
     A.g = p->g;
     A.dt2 = p->dt * p->dt;
     A.dt = p->dt;
@@ -529,6 +587,9 @@ static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float
         A.e_dp = e0;
         A.e_ip = e1;
         A.e_im = e2;
+    } else if (EPI == EPI_BORN_COUPLED) {
+        A.e_dv = e0;
+        A.e_dp = e1;
     }
     A.nzt = (int)((p->nz + ZO - 1) / ZO);
     A.nyt = (int)((p->ny + YO - 1) / YO);
@@ -557,10 +618,16 @@ static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float
     A.cT = f[F_CT].d;
     A.sP = f[F_SP].d;
     A.cP = f[F_CP].d;
-    const long ntiles = (long)A.nzt * A.nyt * nxc;
-    const long nwg = ntiles;
     A.per_xcd = 0;
     A.sync = p->fused_sync;
+    return (long)A.nzt * A.nyt * nxc;
+}
+
+template <int PHYS, int ZW, int NROW, int EPI>
+static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float *e1 = nullptr,
+                                float *e2 = nullptr) {
+    FusedArgs<PHYS> A;
+    const long ntiles = fill_fused<PHYS, ZW, NROW, EPI>(p, linear, e0, e1, e2, A);
     // rounds in step when there are several rounds of one workgroup per CU (the kernel's LDS
     // and registers allow no second one per CU).  It pays where the halo re-reads are heavy
     // (VTI, TTI: 4 and 8 material fields; measured -20 % fetched bytes, -3 % time for TTI
@@ -571,7 +638,7 @@ static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float
         A.per_xcd = p->cu_count / 8;
         WFD_TRY(hipMemsetAsync(p->fused_sync, 0, 8 * sizeof(unsigned), p->stream));
     }
-    k_fused_step<PHYS, ZW, NROW, EPI><<<dim3((unsigned)nwg), dim3(ZW * NROW), 0, p->stream>>>(A);
+    k_fused_step<PHYS, ZW, NROW, EPI><<<dim3((unsigned)ntiles), dim3(ZW * NROW), 0, p->stream>>>(A);
     return true;
 }
 
@@ -584,16 +651,52 @@ static inline bool fused_fits(Prop *p) {
     return true;
 }
 
+// tile shapes of the fused kernel (threads along z x along y; outputs (z-7) x (y-7))
+enum FusedTile { TILE_AUTO = 0, TILE_64x16 = 1, TILE_32x32 = 2 };
+
+static inline bool set_fused_tile(Prop *p, long tile) {
+    if (tile < TILE_AUTO || tile > TILE_32x32) {
+        set_error("SetFusedTile: tile must be 0 (automatic), 1 (64x16) or 2 (32x32)");
+        return false;
+    }
+    p->fused_tile = (int)tile;
+    return true;
+}
+
+// automatic choice (measured on MI355X, 800^3 TTI, 512^3 VTI, 256^3 Iso): the square tile
+// recomputes fewer halo temporaries per output (1024 / 625 threads per output against
+// 1024 / 504) but requests more cache lines per output (its 32-float rows straddle two 128-byte
+// lines); Iso, with one material field, gains (-13 %), VTI and TTI lose (+7 %, +11 %)
+static inline int fused_tile_of(const Prop *p, int epi) {
+    if (p->fused_tile != TILE_AUTO) return p->fused_tile;
+    // the Born epilogues read 1 to 3 more fields per output: the wide tile wins there (Iso Born
+    // passes on 200^3..600^3 windows: 6798 against 6416 shots per hour)
+    return p->phys == ISO && epi == EPI_NONE ? TILE_32x32 : TILE_64x16;
+}
+
+template <int PHYS, int EPI>
+static inline bool launch_fused_tile(Prop *p, bool linear, float *e0, float *e1, float *e2) {
+    if (fused_tile_of(p, EPI) == TILE_32x32) return launch_fused<PHYS, 32, 32, EPI>(p, linear, e0, e1, e2);
+    return launch_fused<PHYS, 64, 16, EPI>(p, linear, e0, e1, e2);
+}
+
 template <int EPI>
 static inline bool step_fused_epi(Prop *p, bool linear, float *e0, float *e1, float *e2) {
     if (!fused_fits(p)) return false;
     bool ok;
-    if (p->phys == ISO)
-        ok = launch_fused<ISO, 64, 16, EPI>(p, linear, e0, e1, e2);
-    else if (p->phys == VTI)
-        ok = launch_fused<VTI, 64, 16, EPI>(p, linear, e0, e1, e2);
-    else
-        ok = launch_fused<TTI, 64, 16, EPI>(p, linear, e0, e1, e2);
+    if constexpr (EPI == EPI_BORN_COUPLED) {
+        if (p->phys != ISO) {
+            set_error("the coupled Born step is defined for Iso only");
+            return false;
+        }
+        ok = launch_fused_tile<ISO, EPI>(p, linear, e0, e1, e2);
+    } else if (p->phys == ISO) {
+        ok = launch_fused_tile<ISO, EPI>(p, linear, e0, e1, e2);
+    } else if (p->phys == VTI) {
+        ok = launch_fused_tile<VTI, EPI>(p, linear, e0, e1, e2);
+    } else {
+        ok = launch_fused_tile<TTI, EPI>(p, linear, e0, e1, e2);
+    }
     if (!ok) return false;
     WFD_LAUNCHED();
     return true;
@@ -601,6 +704,50 @@ static inline bool step_fused_epi(Prop *p, bool linear, float *e0, float *e1, fl
 
 static inline bool step_fused(Prop *p, bool linear) {
     return step_fused_epi<EPI_NONE>(p, linear, nullptr, nullptr, nullptr);
+}
+
+// one launch of k_fused_batch over the n grids, on ps[0]'s stream (the caller orders the
+// other handles' streams around it); e0/e1/e2: per-grid epilogue arrays (device) or null
+template <int PHYS, int ZW, int NROW, int EPI>
+static inline bool launch_fused_batch(Prop **ps, int n, float *const *e0, float *const *e1, float *const *e2) {
+    FusedBatch<PHYS> B = {};
+    B.n = n;
+    long wg = 0;
+    for (int i = 0; i < n; i++) {
+        B.wg[i] = (int)wg;
+        const long t = fill_fused<PHYS, ZW, NROW, EPI>(ps[i], false, e0 ? e0[i] : nullptr, e1 ? e1[i] : nullptr,
+                                                      e2 ? e2[i] : nullptr, B.a[i]);
+        wg += (t + 7) / 8 * 8;
+        if (wg >= (1L << 31)) {
+            set_error("batched step: too many workgroups");
+            return false;
+        }
+    }
+    B.wg[n] = (int)wg;
+    k_fused_batch<PHYS, ZW, NROW, EPI><<<dim3((unsigned)wg), dim3(ZW * NROW), 0, ps[0]->stream>>>(B);
+    return true;
+}
+
+template <int PHYS, int EPI>
+static inline bool launch_fused_batch_tile(Prop **ps, int n, float *const *e0, float *const *e1, float *const *e2) {
+    if (fused_tile_of(ps[0], EPI) == TILE_32x32) return launch_fused_batch<PHYS, 32, 32, EPI>(ps, n, e0, e1, e2);
+    return launch_fused_batch<PHYS, 64, 16, EPI>(ps, n, e0, e1, e2);
+}
+
+template <int EPI>
+static inline bool step_fused_batch(Prop **ps, int n, float *const *e0, float *const *e1, float *const *e2) {
+    for (int i = 0; i < n; i++)
+        if (!fused_fits(ps[i])) return false;
+    bool ok;
+    if (ps[0]->phys == ISO)
+        ok = launch_fused_batch_tile<ISO, EPI>(ps, n, e0, e1, e2);
+    else if (ps[0]->phys == VTI)
+        ok = launch_fused_batch_tile<VTI, EPI>(ps, n, e0, e1, e2);
+    else
+        ok = launch_fused_batch_tile<TTI, EPI>(ps, n, e0, e1, e2);
+    if (!ok) return false;
+    WFD_LAUNCHED();
+    return true;
 }
 
 }  // namespace wfd

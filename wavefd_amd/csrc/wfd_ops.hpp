@@ -436,4 +436,138 @@ static inline bool born_step(Prop *p, int epi, float *a0, float *a1, float *a2) 
     return p->end(w);
 }
 
+// Iso background and Born wavefields stepped together (additive): the same results as
+// TimeStepSaveDP(DP) on the background (pCur/pOld) followed by TimeStepBornInjection_V(dV, DP)
+// on a handle with the same model whose field is the Born field (mCur/mOld, allocated and
+// zeroed at the first call; registered sources with fields bit 2 and receivers with fields 2
+// then address it).  One pass reads the model once for both fields and forms DP in registers.
+static inline bool born_coupled(Prop *p, float *dV, float *DP) {
+    if (p->phys != ISO) {
+        set_error("the coupled Born step is defined for Iso only");
+        return false;
+    }
+    if (!p->born_field) {
+        for (int i : {F_MCUR, F_MOLD}) {
+            WFD_TRY(hipMalloc(&p->f[i].d, sizeof(float) * p->n));
+            WFD_TRY(hipMemsetAsync(p->f[i].d, 0, sizeof(float) * p->n, p->stream));
+            p->f[i].used = true;
+        }
+        p->born_field = true;
+    }
+    if (!p->begin(p->all_mask())) return false;
+    Prop::Arg a[2];
+    WFD_ARG(dV, 0, true, false, a[0]);
+    WFD_ARG(DP, 1, false, true, a[1]);
+    Field *f = p->f;
+    if (p->fused) {
+        // the new fields are in pOld and mOld: swap both pairs
+        if (!step_fused_epi<EPI_BORN_COUPLED>(p, false, a[0].dev, a[1].dev, nullptr)) return false;
+        p->swap_p();
+        std::swap(f[F_MCUR], f[F_MOLD]);
+    } else {
+        // the two-pass step on each field in turn (the Born field through the p slots)
+        if (!step_device(p, false)) return false;
+        k_scale_space_into<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, f[F_V].d, f[F_B].d, f[F_PSPACE].d, nullptr,
+                                                                   a[1].dev, nullptr);
+        WFD_LAUNCHED();
+        std::swap(f[F_PCUR], f[F_MCUR]);
+        std::swap(f[F_POLD], f[F_MOLD]);
+        const bool ok = step_device(p, false);
+        if (ok) {
+            k_iso_inj_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, f[F_V].d, f[F_B].d, a[0].dev, a[1].dev,
+                                                                   f[F_PCUR].d);
+        }
+        std::swap(f[F_PCUR], f[F_MCUR]);  // each pair back in its slots, both already swapped
+        std::swap(f[F_POLD], f[F_MOLD]);
+        if (!ok) return false;
+        WFD_LAUNCHED();
+    }
+    if (!p->finish_args(a, 2)) return false;
+    return p->end(bit(F_PCUR) | (p->fused ? 0 : bit(F_PSPACE)));
+}
+
+// Several handles of one physics stepped together (additive): the same results as calling,
+// on each handle i, TimeStep (epi 0) or the fused Born step epi (1 SAVE_DP, 2 BORN_INJ,
+// 3 IMAGE) with the arrays a0[i], a1[i], a2[i] in born_step's order.  With the fused kernel
+// on for every handle, one launch of k_fused_batch steps them all; it runs on the first
+// handle's stream after the work queued on every handle's stream, and each handle's later
+// work queues after it.  Handles must be distinct and in DEVICE coherence mode, the arrays
+// device memory.
+static inline bool time_step_batch(Physics phys, void **hs, long n, long epi, float **a0, float **a1,
+                                   float **a2) {
+    if (!hs || n < 1 || n > FUSED_MAX_BATCH) {
+        set_error("TimeStepBatch: between 1 and " + std::to_string(FUSED_MAX_BATCH) + " handles");
+        return false;
+    }
+    if (epi < EPI_NONE || epi > EPI_IMAGE) {
+        set_error("TimeStepBatch: epilogue must be 0 (none), 1 (save DP), 2 (Born injection V) or 3 (imaging V)");
+        return false;
+    }
+    Prop *ps[FUSED_MAX_BATCH];
+    bool all_fused = true;
+    for (long i = 0; i < n; i++) {
+        ps[i] = reinterpret_cast<Prop *>(hs[i]);
+        if (!ps[i] || ps[i]->phys != phys) {
+            set_error("TimeStepBatch: null handle or a handle of another physics");
+            return false;
+        }
+        for (long j = 0; j < i; j++)
+            if (ps[j] == ps[i]) {
+                set_error("TimeStepBatch: a handle appears twice");
+                return false;
+            }
+        if (ps[i]->coherence != DEVICE) {
+            set_error("TimeStepBatch: handles must be in DEVICE coherence mode");
+            return false;
+        }
+        all_fused = all_fused && ps[i]->fused;
+    }
+    const bool two = phys != ISO;
+    // arrays the epilogue needs: SAVE_DP (DP[, DM]) = a0[, a1]; others (x, DP[, DM]) = a0, a1[, a2]
+    const int need = epi == EPI_NONE ? 0 : (epi == EPI_SAVE_DP ? 1 : 2) + (two ? 1 : 0);
+    float **arr[3] = {a0, a1, a2};
+    for (int k = 0; k < need; k++) {
+        if (!arr[k]) {
+            set_error("TimeStepBatch: missing array list");
+            return false;
+        }
+        for (long i = 0; i < n; i++)
+            if (!arr[k][i] || !is_device_ptr(arr[k][i])) {
+                set_error("TimeStepBatch: epilogue arrays must be device memory");
+                return false;
+            }
+    }
+    auto get = [&](int k, long i) -> float * { return k < need ? arr[k][i] : nullptr; };
+    if (!all_fused) {  // the same results, one handle after the other
+        for (long i = 0; i < n; i++) {
+            const bool ok = epi == EPI_NONE ? time_step(ps[i], false)
+                                            : born_step(ps[i], (int)epi, get(0, i), get(1, i), get(2, i));
+            if (!ok) return false;
+        }
+        return true;
+    }
+    hipStream_t s0 = ps[0]->stream;
+    for (long i = 0; i < n; i++)
+        if (!ps[i]->ev) WFD_TRY(hipEventCreateWithFlags(&ps[i]->ev, hipEventDisableTiming));
+    for (long i = 1; i < n; i++)
+        if (ps[i]->stream != s0) {
+            WFD_TRY(hipEventRecord(ps[i]->ev, ps[i]->stream));
+            WFD_TRY(hipStreamWaitEvent(s0, ps[i]->ev, 0));
+        }
+    float *e[3][FUSED_MAX_BATCH] = {};
+    for (int k = 0; k < need; k++)
+        for (long i = 0; i < n; i++) e[k][i] = arr[k][i];
+    const int nn = (int)n;
+    const bool ok = epi == EPI_NONE      ? step_fused_batch<EPI_NONE>(ps, nn, nullptr, nullptr, nullptr)
+                    : epi == EPI_SAVE_DP ? step_fused_batch<EPI_SAVE_DP>(ps, nn, e[0], e[1], nullptr)
+                    : epi == EPI_BORN_INJ ? step_fused_batch<EPI_BORN_INJ>(ps, nn, e[0], e[1], e[2])
+                                          : step_fused_batch<EPI_IMAGE>(ps, nn, e[0], e[1], e[2]);
+    if (!ok) return false;
+    WFD_TRY(hipEventRecord(ps[0]->ev, s0));
+    for (long i = 1; i < n; i++)
+        if (ps[i]->stream != s0) WFD_TRY(hipStreamWaitEvent(ps[i]->stream, ps[0]->ev, 0));
+    for (long i = 0; i < n; i++) ps[i]->swap_p();
+    return true;
+}
+
 }  // namespace wfd

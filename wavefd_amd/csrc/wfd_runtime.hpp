@@ -177,8 +177,11 @@ struct Prop {
     long src_nt = 0, rcv_nt = 0, src_nr = 0, rcv_nr = 0;
     int src_fields = 1, rcv_fields = 1;
     int fused = 1;  // use the fused single-pass step kernels
+    int fused_tile = 0;  // fused kernel tile: 0 automatic, 1 64 (z) x 16 (y), 2 32 x 32
     int cu_count = 256;  // compute units of the device (sizes the fused kernel's x-chunks)
     unsigned *fused_sync = nullptr;  // per-XCD round counters of the persistent fused launch
+    hipEvent_t ev = nullptr;         // orders this handle's stream against a batch's (lazy)
+    bool born_field = false;         // Iso: mCur/mOld hold the coupled Born step's second field
 
     Mask model_mask() const {
         Mask m = bit(F_V) | bit(F_B) | bit(F_Q);
@@ -220,6 +223,7 @@ struct Prop {
     ~Prop() {
         if (stream) hipStreamSynchronize(stream);
         if (fused_sync) (void)hipFree(fused_sync);
+        if (ev) (void)hipEventDestroy(ev);
         for (int i = 0; i < F_COUNT; i++) {
             if (f[i].d) hipFree(f[i].d);
             if (f[i].h) hipHostFree(f[i].h);
@@ -361,6 +365,7 @@ static inline Prop *make_prop(Physics phys, long fs, long nthread, long nx, long
     p->g.idy = (float)(1.0 / dy);
     p->g.idz = (float)(1.0 / dz);
     if (const char *e = std::getenv("WAVEFD_FUSED")) p->fused = std::atoi(e);
+    if (const char *e = std::getenv("WAVEFD_FUSED_TILE")) p->fused_tile = std::atoi(e);
     if (!p->init()) {
         delete p;
         return nullptr;
@@ -448,7 +453,7 @@ static inline bool inject_registered(Prop *p, long it) {
                                                    p->src.gstart, p->src.ngroups);
         WFD_LAUNCHED();
     }
-    if ((p->src_fields & 2) && p->phys != ISO) {
+    if ((p->src_fields & 2) && (p->phys != ISO || p->born_field)) {
         k_inject_groups<<<nb, 256, 0, p->stream>>>(p->f[F_MCUR].d, p->src_data, it, p->src_nt, p->src.pts,
                                                    p->src.gstart, p->src.ngroups);
         WFD_LAUNCHED();
@@ -459,7 +464,7 @@ static inline bool inject_registered(Prop *p, long it) {
 static inline bool extract_registered(Prop *p, long it) {
     if (!p->rcv.valid()) return true;
     const int nb = (p->rcv.ngroups + 255) / 256;
-    const int fld = (p->rcv_fields == 2 && p->phys != ISO) ? F_MCUR : F_PCUR;
+    const int fld = (p->rcv_fields == 2 && (p->phys != ISO || p->born_field)) ? F_MCUR : F_PCUR;
     k_extract_groups<<<nb, 256, 0, p->stream>>>(p->rcv_data, p->f[fld].d, it, p->rcv_nt, p->rcv.pts, p->rcv.gstart,
                                                 p->rcv.ngroups);
     WFD_LAUNCHED();

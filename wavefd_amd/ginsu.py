@@ -161,6 +161,13 @@ def copy(dst, src, nbytes, stream=None):
         raise RuntimeError(_lib.last_error("ginsu"))
 
 
+def sync(stream=None):
+    """wait for the work queued on a stream (None: the null stream)"""
+    lib = _lib.load("ginsu")
+    if lib.wavefd_device_sync(stream) != 0:
+        raise RuntimeError(_lib.last_error("ginsu"))
+
+
 def zero(dst, nbytes, stream=None):
     lib = _lib.load("ginsu")
     if lib.wavefd_device_zero(int(dst), int(nbytes), stream) != 0:

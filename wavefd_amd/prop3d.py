@@ -215,6 +215,16 @@ def set_fused_step(prop, fused=True):
     prop._lib.SetFusedStep(prop.p, 1 if fused else 0)
 
 
+FUSED_TILES = {"auto": 0, "64x16": 1, "32x32": 2}
+
+
+def set_fused_tile(prop, tile="auto"):
+    """Tile shape of the fused step kernel: "64x16" (64 z-lanes x 16 y-rows per workgroup,
+    56 x 9 outputs), "32x32" (25 x 25 outputs) or "auto".  Results do not depend on it."""
+    if prop._lib.SetFusedTile(prop.p, FUSED_TILES[tile]) != 0:
+        prop._check()
+
+
 def _partitions_args(partitions):
     from .spacetime import _partition_arrays
     return _partition_arrays(partitions)
