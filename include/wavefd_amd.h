@@ -48,6 +48,17 @@ struct SourcePoint32; /* spacetime.h */
      * fields: 1 = P, 2 = M.  0 on success. */                                                     \
     long P##_SetReceivers(void *p, struct SourcePoint32 **partitions, size_t npartitions, long *npoints, \
                           size_t nt, long fields);                                                  \
+    /* the same with the time interpolation fused in: the source data (Julia (nd, nr)) is at the  \
+     * coarse rate, and each injection at fine step it adds the reference's interpadjoint of it   \
+     * (src/spacetime.cc:4-62); the receivers' trace buffer (nd, nr) is at the coarse rate, and   \
+     * each extraction at fine step it adds the interpforward contributions of that fine sample  \
+     * (src/spacetime.cc:64-127).  h: nh x 8 filters (interpfilters, src/spacetime.jl:333-364), \
+     * host or device; nd >= 7 coarse samples, nm fine samples.  The results equal the host       \
+     * interpolation followed by the fine-rate calls, bit for bit.  0 on success. */             \
+    long P##_SetSourcesInterp(void *p, struct SourcePoint32 **partitions, size_t npartitions, long *npoints, \
+                              float *data, size_t nd, float *h, size_t nh, size_t nm, long fields);  \
+    long P##_SetReceiversInterp(void *p, struct SourcePoint32 **partitions, size_t npartitions, long *npoints, \
+                                size_t nd, float *h, size_t nh, size_t nm, long fields);            \
     /* TimeStep, then inject the sources at it (1-based) into the new pCur (mCur), then extract   \
      * the receivers at it from the new pCur: the loop body of test/spacetime.jl:486-492 */       \
     void P##_TimeStepInjectExtract(void *p, long it);                                               \

@@ -13,7 +13,7 @@ from .prop3d import (  # noqa: F401
     Prop3DAcoTTIDenQ_DEO2_FDTD_Model_VEA, Prop3DAcoVTIDenQ_DEO2_FDTD, Prop3DAcoVTIDenQ_DEO2_FDTD_Model_V,
     Prop3DAcoVTIDenQ_DEO2_FDTD_Model_VEA, SinPhi, SinTheta, V, adjointBornAccumulation, coherence,
     device_ptr, forwardBornInjection, free, propagateadjoint, propagateforward, propagateforward_linear,
-    receiver_data, scale_spatial_derivatives, set_fused_step, set_fused_tile, set_receivers, set_sources, size,
+    receiver_data, scale_spatial_derivatives, set_fused_step, set_fused_tile, set_receivers, set_receivers_interp, set_sources, set_sources_interp, size,
     sync_to_device, sync_to_host, synchronize, timestep_inject_extract)
 from .spacetime import (  # noqa: F401
     SourcePoint32, causal_ricker, extractdata, hickscoeffs, injectdata, interpadjoint, interpfilters,

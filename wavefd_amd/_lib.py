@@ -38,6 +38,8 @@ _COMMON = {
     "GetStream": (_P, [_P]),
     "SetSources": (_L, [_P, _P, _S, _P, _P, _S, _L]),
     "SetReceivers": (_L, [_P, _P, _S, _P, _S, _L]),
+    "SetSourcesInterp": (_L, [_P, _P, _S, _P, _P, _S, _P, _S, _S, _L]),
+    "SetReceiversInterp": (_L, [_P, _P, _S, _P, _S, _P, _S, _S, _L]),
     "InjectRegistered": (_L, [_P, _L]),
     "ExtractRegistered": (_L, [_P, _L]),
     "TimeStepInjectExtract": (None, [_P, _L]),

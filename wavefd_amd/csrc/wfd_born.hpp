@@ -202,4 +202,66 @@ __global__ void k_extract_groups(float *__restrict__ data, const float *__restri
     data[id] = acc;
 }
 
+// ---- registered points with the time interpolation fused in -----------------------------
+// The reference's 8-tap polyphase interpolation (src/spacetime.cc:4-127) maps a coarse trace d
+// (nd samples) and a fine one m (nm samples, nh fine per coarse) through filters h[k][0..7].
+// Fine sample jk = i*nh + k belongs to row i; the row's taps l in [lo, hi] pair it with coarse
+// samples i-3+l, the ranges trimmed at both ends: (0,[3,7]) (1,[2,7]) (2,[1,7]) (3..nd-5,[0,7])
+// (nd-4,[0,6]) (nd-3,[0,5]) (nd-2,[0,4]); rows past nd-2 are never visited (nd >= 7 here, where
+// every row is visited once).
+//   interpadjoint (coarse wavelet -> fine): m[jk] = sum over l = lo..hi of h[k][l] d[i-3+l],
+//     accumulated from 0 in l order -- computed for the injected step alone;
+//   interpforward (fine traces -> coarse): d[i-3+l] += h[k][l] m[jk] for l = lo..hi, the
+//     reference visiting rows, then k, then l; for one coarse sample that is fine-time order, so
+//     adding each step's contributions as the step is extracted gives the reference's sums.
+__device__ __forceinline__ bool interp_row(long jk, long nh, long nd, long nm, long &i, int &k, int &lo, int &hi) {
+    if (jk < 0 || jk >= nm) return false;
+    i = jk / nh;
+    k = (int)(jk - i * nh);
+    if (i > nd - 2) return false;
+    lo = i == 0 ? 3 : (i == 1 ? 2 : (i == 2 ? 1 : 0));
+    hi = i == nd - 4 ? 6 : (i == nd - 3 ? 5 : (i == nd - 2 ? 4 : 7));
+    return true;
+}
+
+// injection at fine step it (1-based) of coarse data (nr x nd) through the filters h (nh x 8)
+__global__ void k_inject_groups_interp(float *__restrict__ field, const float *__restrict__ data, long it, long nd,
+                                       const float *__restrict__ h, long nh, long nm,
+                                       const SourcePoint32 *__restrict__ pts, const int *__restrict__ gstart,
+                                       int ngroups) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ngroups) return;
+    long i;
+    int k, lo, hi;
+    if (!interp_row(it - 1, nh, nd, nm, i, k, lo, hi)) return;  // the fine sample is 0: nothing to add
+    const int s = gstart[gi], e = gstart[gi + 1];
+    const long iu = pts[s].iu - 1;
+    float acc = field[iu];
+    for (int q = s; q < e; q++) {
+        const float *d = data + (pts[q].ir - 1) * nd + (i - 3);
+        float m = 0.0f;
+        for (int l = lo; l <= hi; l++) m += h[k * 8 + l] * d[l];
+        acc += pts[q].c * m;
+    }
+    field[iu] = acc;
+}
+
+// extraction at fine step it into coarse traces (nr x nd): the fine value of the trace, then
+// its interpforward contributions
+__global__ void k_extract_groups_interp(float *__restrict__ data, const float *__restrict__ field, long it, long nd,
+                                        const float *__restrict__ h, long nh, long nm,
+                                        const SourcePoint32 *__restrict__ pts, const int *__restrict__ gstart,
+                                        int ngroups) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ngroups) return;
+    long i;
+    int k, lo, hi;
+    if (!interp_row(it - 1, nh, nd, nm, i, k, lo, hi)) return;
+    const int s = gstart[gi], e = gstart[gi + 1];
+    float v = 0.0f;
+    for (int q = s; q < e; q++) v += pts[q].c * field[pts[q].iu - 1];
+    float *d = data + (pts[s].ir - 1) * nd + (i - 3);
+    for (int l = lo; l <= hi; l++) d[l] += h[k * 8 + l] * v;
+}
+
 }  // namespace wfd

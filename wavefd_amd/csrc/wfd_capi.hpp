@@ -79,6 +79,17 @@
                                        size_t nt, long fields) {                                          \
         return wfd::set_receivers(reinterpret_cast<wfd::Prop *>(p), parts, nparts, npts, nt, fields) ? 0 : -1; \
     }                                                                                                     \
+    long WFD_CAT(Prefix, SetSourcesInterp)(void *p, wfd::SourcePoint32 **parts, size_t nparts, long *npts, \
+                                           float *data, size_t nd, float *h, size_t nh, size_t nm,          \
+                                           long fields) {                                                   \
+        return wfd::set_sources_interp(reinterpret_cast<wfd::Prop *>(p), parts, nparts, npts, data, nd, h, nh, \
+                                       nm, fields) ? 0 : -1;                                                \
+    }                                                                                                     \
+    long WFD_CAT(Prefix, SetReceiversInterp)(void *p, wfd::SourcePoint32 **parts, size_t nparts, long *npts, \
+                                             size_t nd, float *h, size_t nh, size_t nm, long fields) {        \
+        return wfd::set_receivers_interp(reinterpret_cast<wfd::Prop *>(p), parts, nparts, npts, nd, h, nh, nm, \
+                                         fields) ? 0 : -1;                                                  \
+    }                                                                                                     \
     void WFD_CAT(Prefix, TimeStepInjectExtract)(void *p, long it) {                                       \
         wfd::time_step_inject_extract(reinterpret_cast<wfd::Prop *>(p), it, false);                       \
     }                                                                                                     \

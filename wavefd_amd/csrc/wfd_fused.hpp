@@ -631,9 +631,11 @@ static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float
     // rounds in step when there are several rounds of one workgroup per CU (the kernel's LDS
     // and registers allow no second one per CU).  It pays where the halo re-reads are heavy
     // (VTI, TTI: 4 and 8 material fields; measured -20 % fetched bytes, -3 % time for TTI
-    // 800^3); for Iso the waiting costs more than the shared lines save.
-    bool lockstep = PHYS != ISO && p->cu_count % 8 == 0 && ntiles >= 2L * p->cu_count;
-    if (const char *e = std::getenv("WAVEFD_FUSED_LOCKSTEP")) lockstep = lockstep && std::atoi(e) != 0;
+    // 800^3); for Iso, and for the Born epilogues (VTI Born passes on 200^3..600^3 windows:
+    // 6876 against 7032 shots per hour), the waiting costs more than the shared lines save.
+    const bool can = p->cu_count % 8 == 0 && ntiles >= 2L * p->cu_count;
+    bool lockstep = can && PHYS != ISO && EPI == EPI_NONE;
+    if (const char *e = std::getenv("WAVEFD_FUSED_LOCKSTEP")) lockstep = can && std::atoi(e) != 0;
     if (lockstep && p->fused_sync) {
         A.per_xcd = p->cu_count / 8;
         WFD_TRY(hipMemsetAsync(p->fused_sync, 0, 8 * sizeof(unsigned), p->stream));

@@ -175,6 +175,10 @@ struct Prop {
     const float *src_data = nullptr;  // device copy of the source data (nr x nt)
     float *rcv_data = nullptr;        // device trace buffer (nr x nt)
     long src_nt = 0, rcv_nt = 0, src_nr = 0, rcv_nr = 0;
+    // fused time interpolation (nh > 0): data at the coarse rate (src_nt / rcv_nt coarse
+    // samples), filters h (nh x 8) on the device, nm fine samples
+    float *src_h = nullptr, *rcv_h = nullptr;
+    long src_nh = 0, rcv_nh = 0, src_nm = 0, rcv_nm = 0;
     int src_fields = 1, rcv_fields = 1;
     int fused = 1;  // use the fused single-pass step kernels
     int fused_tile = 0;  // fused kernel tile: 0 automatic, 1 64 (z) x 16 (y), 2 32 x 32
@@ -236,6 +240,8 @@ struct Prop {
         free_pointset(rcv);
         if (src_data) hipFree((void *)src_data);
         if (rcv_data) hipFree(rcv_data);
+        if (src_h) hipFree(src_h);
+        if (rcv_h) hipFree(rcv_h);
         if (stream) hipStreamDestroy(stream);
     }
 
@@ -448,14 +454,20 @@ static inline bool rminus(Prop *p, const float *i0, const float *i1, const float
 static inline bool inject_registered(Prop *p, long it) {
     if (!p->src.valid()) return true;
     const int nb = (p->src.ngroups + 255) / 256;
+    auto inject = [&](float *field) {
+        if (p->src_nh > 0)
+            k_inject_groups_interp<<<nb, 256, 0, p->stream>>>(field, p->src_data, it, p->src_nt, p->src_h, p->src_nh,
+                                                              p->src_nm, p->src.pts, p->src.gstart, p->src.ngroups);
+        else
+            k_inject_groups<<<nb, 256, 0, p->stream>>>(field, p->src_data, it, p->src_nt, p->src.pts, p->src.gstart,
+                                                       p->src.ngroups);
+    };
     if (p->src_fields & 1) {
-        k_inject_groups<<<nb, 256, 0, p->stream>>>(p->f[F_PCUR].d, p->src_data, it, p->src_nt, p->src.pts,
-                                                   p->src.gstart, p->src.ngroups);
+        inject(p->f[F_PCUR].d);
         WFD_LAUNCHED();
     }
     if ((p->src_fields & 2) && (p->phys != ISO || p->born_field)) {
-        k_inject_groups<<<nb, 256, 0, p->stream>>>(p->f[F_MCUR].d, p->src_data, it, p->src_nt, p->src.pts,
-                                                   p->src.gstart, p->src.ngroups);
+        inject(p->f[F_MCUR].d);
         WFD_LAUNCHED();
     }
     return true;
@@ -465,8 +477,13 @@ static inline bool extract_registered(Prop *p, long it) {
     if (!p->rcv.valid()) return true;
     const int nb = (p->rcv.ngroups + 255) / 256;
     const int fld = (p->rcv_fields == 2 && (p->phys != ISO || p->born_field)) ? F_MCUR : F_PCUR;
-    k_extract_groups<<<nb, 256, 0, p->stream>>>(p->rcv_data, p->f[fld].d, it, p->rcv_nt, p->rcv.pts, p->rcv.gstart,
-                                                p->rcv.ngroups);
+    if (p->rcv_nh > 0)
+        k_extract_groups_interp<<<nb, 256, 0, p->stream>>>(p->rcv_data, p->f[fld].d, it, p->rcv_nt, p->rcv_h,
+                                                           p->rcv_nh, p->rcv_nm, p->rcv.pts, p->rcv.gstart,
+                                                           p->rcv.ngroups);
+    else
+        k_extract_groups<<<nb, 256, 0, p->stream>>>(p->rcv_data, p->f[fld].d, it, p->rcv_nt, p->rcv.pts,
+                                                    p->rcv.gstart, p->rcv.ngroups);
     WFD_LAUNCHED();
     return true;
 }
@@ -477,6 +494,7 @@ static inline bool set_sources(Prop *p, SourcePoint32 **parts, size_t nparts, co
     if (p->src_data) hipFree((void *)p->src_data);
     p->src_data = nullptr;
     p->src_nt = (long)nt;
+    p->src_nh = p->src_nm = 0;
     p->src_fields = (int)fields;
     p->src_nr = p->src.nr;
     if (!p->src.valid()) return true;
@@ -496,6 +514,7 @@ static inline bool set_receivers(Prop *p, SourcePoint32 **parts, size_t nparts, 
     if (p->rcv_data) hipFree(p->rcv_data);
     p->rcv_data = nullptr;
     p->rcv_nt = (long)nt;
+    p->rcv_nh = p->rcv_nm = 0;
     p->rcv_fields = (int)fields;
     p->rcv_nr = p->rcv.nr;
     if (!p->rcv.valid()) return true;
@@ -504,6 +523,41 @@ static inline bool set_receivers(Prop *p, SourcePoint32 **parts, size_t nparts, 
     WFD_TRY(hipMemsetAsync(p->rcv_data, 0, bytes, p->stream));
     WFD_TRY(hipStreamSynchronize(p->stream));
     return true;
+}
+
+// the filters of a fused interpolation: nd coarse samples (at least 7), nh >= 1 fine samples
+// per coarse one, nm fine samples; h (nh x 8, host or device) copied to the device
+static inline bool set_interp(Prop *p, long nd, const float *h, long nh, long nm, float *&dst_h, long &dst_nh,
+                              long &dst_nm) {
+    if (nd < 7 || nh < 1 || nm < 1 || !h) {
+        set_error("interpolated points: need nd >= 7 coarse samples, nh >= 1, nm >= 1 and filters h");
+        return false;
+    }
+    if (dst_h) hipFree(dst_h);
+    dst_h = nullptr;
+    WFD_TRY(hipMalloc(&dst_h, sizeof(float) * 8 * nh));
+    WFD_TRY(hipMemcpyAsync(dst_h, h, sizeof(float) * 8 * nh, is_device_ptr(h) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                           p->stream));
+    WFD_TRY(hipStreamSynchronize(p->stream));
+    dst_nh = nh;
+    dst_nm = nm;
+    return true;
+}
+
+// sources whose data (nr x nd) is at the coarse rate: each injection at fine step it adds the
+// reference's interpadjoint of the data at it (src/spacetime.cc:4-62)
+static inline bool set_sources_interp(Prop *p, SourcePoint32 **parts, size_t nparts, const long *npts,
+                                      const float *data, size_t nd, const float *h, size_t nh, size_t nm, long fields) {
+    if (!set_sources(p, parts, nparts, npts, data, nd, fields)) return false;
+    return set_interp(p, (long)nd, h, (long)nh, (long)nm, p->src_h, p->src_nh, p->src_nm);
+}
+
+// receivers with a coarse trace buffer (nr x nd, zeroed): each extraction at fine step it adds
+// the reference's interpforward contributions of the fine sample (src/spacetime.cc:64-127)
+static inline bool set_receivers_interp(Prop *p, SourcePoint32 **parts, size_t nparts, const long *npts, size_t nd,
+                                        const float *h, size_t nh, size_t nm, long fields) {
+    if (!set_receivers(p, parts, nparts, npts, nd, fields)) return false;
+    return set_interp(p, (long)nd, h, (long)nh, (long)nm, p->rcv_h, p->rcv_nh, p->rcv_nm);
 }
 
 static inline bool get_receiver_data(Prop *p, float *out) {

@@ -253,6 +253,34 @@ def set_receivers(prop, partitions, nt, fields=1):
     prop._rcv_shape = None
 
 
+def set_sources_interp(prop, partitions, data, h, nm, fields=1):
+    """Register injection points whose data ((nd, nr) Julia layout == (nr, nd) C-order float32)
+    is at the coarse rate: each injection at fine step it adds the reference's interpadjoint of
+    the data at it (filters h = interpfilters(dtmod, dtrec), nm fine samples)."""
+    keep, parts_pp, nparts, npts = _partitions_args(partitions)
+    data = np.ascontiguousarray(data, np.float32)
+    if data.ndim != 2:
+        raise ValueError("data must be a 2D (nr, nd) C-ordered float32 array")
+    h = np.ascontiguousarray(h, np.float32)
+    rc = prop._lib.SetSourcesInterp(prop.p, parts_pp, nparts, npts, data.ctypes.data, data.shape[1], h.ctypes.data,
+                                    h.shape[0], nm, fields)
+    if rc != 0:
+        prop._check()
+    prop._src_keep = keep
+
+
+def set_receivers_interp(prop, partitions, nd, h, nm, fields=1):
+    """Register extraction points with a coarse (nr, nd) trace buffer: each extraction at fine
+    step it adds the reference's interpforward contributions of that fine sample."""
+    keep, parts_pp, nparts, npts = _partitions_args(partitions)
+    h = np.ascontiguousarray(h, np.float32)
+    rc = prop._lib.SetReceiversInterp(prop.p, parts_pp, nparts, npts, nd, h.ctypes.data, h.shape[0], nm, fields)
+    if rc != 0:
+        prop._check()
+    prop._rcv_keep = keep
+    prop._rcv_shape = None
+
+
 def timestep_inject_extract(prop, it):
     """propagateforward, then inject the registered sources at it (1-based) into the new
     pcur, then extract the registered receivers at it."""
