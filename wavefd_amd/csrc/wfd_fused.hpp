@@ -632,8 +632,10 @@ static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float
     // and registers allow no second one per CU).  It pays where the halo re-reads are heavy
     // (VTI, TTI: 4 and 8 material fields; measured -20 % fetched bytes, -3 % time for TTI
     // 800^3); for Iso, and for the Born epilogues (VTI Born passes on 200^3..600^3 windows:
-    // 6876 against 7032 shots per hour), the waiting costs more than the shared lines save.
-    const bool can = p->cu_count % 8 == 0 && ntiles >= 2L * p->cu_count;
+    // 6876 against 7032 shots per hour), the waiting costs more than the shared lines save.  It
+    // needs many rounds: 800^3 (21 rounds) TTI 13.31 -> 12.73 ms, VTI 10.36 -> 10.14 ms; 512^3
+    // (9 rounds) TTI 3.43 -> 3.70 ms, VTI 2.79 -> 2.99 ms.
+    const bool can = p->cu_count % 8 == 0 && ntiles >= 16L * p->cu_count;
     bool lockstep = can && PHYS != ISO && EPI == EPI_NONE;
     if (const char *e = std::getenv("WAVEFD_FUSED_LOCKSTEP")) lockstep = can && std::atoi(e) != 0;
     if (lockstep && p->fused_sync) {
