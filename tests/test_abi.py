@@ -21,6 +21,7 @@ HEADER_OF = {
     "spacetime": "spacetime.h",
     "codec": "wavefd_codec.h",
     "ginsu": "wavefd_ginsu.h",
+    "lcodec": "wavefd_lcodec.h",
 }
 
 REFERENCE_SYMBOLS = {
@@ -66,7 +67,7 @@ def exported(path):
     return {line.split()[-1] for line in out.splitlines() if " T " in line}
 
 
-ALL = ["iso", "vti", "tti", "spacetime", "codec", "ginsu"]
+ALL = ["iso", "vti", "tti", "spacetime", "codec", "ginsu", "lcodec"]
 
 
 @pytest.mark.parametrize("kind", ALL)

@@ -124,14 +124,39 @@ GINSU = {
 }
 
 
+LCODEC = {
+    "wavefd_lcodec_code_words": (_L, [_L, _L, _L]),
+    "wavefd_lcodec_encode": (_L, [_P, _L, _L, _L, _P, _P]),
+    "wavefd_lcodec_dequantize": (_L, [_P, _P, _P]),
+    "wavefd_lcodec_nparams": (_L, []),
+    "wavefd_lcodec_net_create": (_P, []),
+    "wavefd_lcodec_net_free": (None, [_P]),
+    "wavefd_lcodec_net_get": (_L, [_P, _P]),
+    "wavefd_lcodec_net_set": (_L, [_P, _P]),
+    "wavefd_lcodec_net_init": (_L, [_P, ctypes.c_ulong]),
+    "wavefd_lcodec_forward": (_L, [_P, _P, _L, _L, _L, _L, ctypes.c_float, _P, _P, _P]),
+    "wavefd_lcodec_grad": (_L, [_P, _P, _P, _L, _L, _L, _L, ctypes.c_float, _P, _P, _P, _P, _P]),
+    "wavefd_lcodec_halfstep": (_L, [_P, _P, _P]),
+    "wavefd_lcodec_encode_learned": (_L, [_P, _P, _L, _L, _L, _P, _P, _P, _P]),
+    "wavefd_lcodec_adam": (_L, [_P, ctypes.c_float, _P]),
+    "wavefd_lcodec_decode": (_L, [_P, _P, _P, _P, _P]),
+    "wavefd_lcodec_gather": (_L, [_P, _L, _L, _L, _P, _L, _L, ctypes.c_float, _P, _P]),
+    "wavefd_lcodec_norm": (ctypes.c_float, [_P]),
+    "wavefd_lcodec_LastError": (ctypes.c_char_p, []),
+}
+
+_TABLES = {"spacetime": SPACETIME, "codec": CODEC, "ginsu": GINSU, "lcodec": LCODEC}
+_LIBNAMES = {"spacetime": "libspacetime.so", "codec": "libwavefd_codec.so", "ginsu": "libwavefd_ginsu.so",
+             "lcodec": "libwavefd_lcodec.so"}
+_ERRFN = {"spacetime": "spacetime_LastError", "codec": "wavefd_codec_LastError",
+          "ginsu": "wavefd_ginsu_LastError", "lcodec": "wavefd_lcodec_LastError"}
+
+
 def signatures(kind):
-    """{symbol: (restype, argtypes)} for 'iso' | 'vti' | 'tti' | 'spacetime' | 'codec' | 'ginsu'."""
-    if kind == "spacetime":
-        return dict(SPACETIME)
-    if kind == "codec":
-        return dict(CODEC)
-    if kind == "ginsu":
-        return dict(GINSU)
+    """{symbol: (restype, argtypes)} for 'iso' | 'vti' | 'tti' | 'spacetime' | 'codec' | 'ginsu' |
+    'lcodec'."""
+    if kind in _TABLES:
+        return dict(_TABLES[kind])
     pre = PREFIXES[kind]
     out = {}
     for name, sig in list(_COMMON.items()) + list(_PHYS[kind].items()):
@@ -140,12 +165,8 @@ def signatures(kind):
 
 
 def lib_path(kind):
-    if kind == "spacetime":
-        name = "libspacetime.so"
-    elif kind == "codec":
-        name = "libwavefd_codec.so"
-    elif kind == "ginsu":
-        name = "libwavefd_ginsu.so"
+    if kind in _LIBNAMES:
+        name = _LIBNAMES[kind]
     else:  # the reference's library names start in lower case: libprop3DAco...
         pre = PREFIXES[kind]
         name = "lib" + pre[0].lower() + pre[1:] + ".so"
@@ -189,12 +210,5 @@ def load(kind):
 
 def last_error(kind):
     lib = load(kind)
-    if kind == "spacetime":
-        msg = lib.spacetime_LastError()
-    elif kind == "codec":
-        msg = lib.wavefd_codec_LastError()
-    elif kind == "ginsu":
-        msg = lib.wavefd_ginsu_LastError()
-    else:
-        msg = lib.LastError()
+    msg = getattr(lib, _ERRFN[kind])() if kind in _ERRFN else lib.LastError()
     return msg.decode() if msg else ""

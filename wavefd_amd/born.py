@@ -70,6 +70,72 @@ class _DevArray:
         return self.ptr
 
 
+class _ExactStore:
+    """DP_n (DM_n) kept exactly: one float32 field per step and wavefield"""
+
+    def __init__(self, ncell, nt, nf, buffer=None):
+        self.ncell, self.nf = ncell, nf
+        self.own = buffer is None
+        self.buf = ginsu.DeviceBuffer(4 * ncell * nt * nf) if buffer is None else buffer
+        if self.buf.nbytes < 4 * ncell * nt * nf:
+            raise ValueError("snapshot buffer too small")
+
+    def slot(self, kind, n):
+        return self.buf.ptr + 4 * self.ncell * ((n - 1) * self.nf + kind)
+
+    def out(self, kind, n):  # where the step writes DP_n
+        return self.slot(kind, n)
+
+    def commit(self, kind, n, stream):
+        pass
+
+    def get(self, kind, n, stream):  # where DP_n can be read
+        return self.slot(kind, n)
+
+    def free(self):
+        if self.own:
+            self.buf.free()
+
+
+class _CodecStore:
+    """DP_n (DM_n) kept as learned-codec codes (lcodec): written to a scratch field, encoded;
+    read back through the learned decoder (or the plain dequantisation, learned=False)"""
+
+    def __init__(self, shape, nt, nf, codec, learned=True):
+        from . import lcodec
+        self.lc, self.codec, self.learned, self.shape, self.nf = lcodec, codec, learned, shape, nf
+        self.words = lcodec.code_words(shape)
+        ncell = int(np.prod(shape))
+        self.codes = ginsu.DeviceBuffer(4 * self.words * nt * nf)
+        self.scratch = [ginsu.DeviceBuffer(4 * ncell) for _ in range(nf)]
+        self.tmp = ginsu.DeviceBuffer(4 * ncell)
+        self.tmp2 = ginsu.DeviceBuffer(4 * ncell) if learned else None
+
+    def code(self, kind, n):
+        return self.codes.ptr + 4 * self.words * ((n - 1) * self.nf + kind)
+
+    def out(self, kind, n):
+        return self.scratch[kind].ptr
+
+    def commit(self, kind, n, stream):
+        if self.learned:  # with the encoder's per-block choice of the learned decode
+            self.codec.encode(self.scratch[kind].ptr, self.shape, self.code(kind, n), self.tmp.ptr, self.tmp2.ptr,
+                              stream)
+        else:
+            self.lc.encode(self.scratch[kind].ptr, self.shape, self.code(kind, n), stream)
+
+    def get(self, kind, n, stream):
+        if self.learned:
+            self.codec.decode(self.code(kind, n), self.scratch[kind].ptr, self.tmp.ptr, stream)
+        else:
+            self.lc.dequantize(self.code(kind, n), self.scratch[kind].ptr, stream)
+        return self.scratch[kind].ptr
+
+    def free(self):
+        for b in [self.codes, self.tmp] + self.scratch + ([self.tmp2] if self.tmp2 else []):
+            b.free()
+
+
 class BornShot:
     """One shot: source (z, y, x), receiver coordinate arrays, a wavelet of nt samples at dt, all
     in the big grid's coordinates (origin 0)."""
@@ -89,7 +155,7 @@ class _ShotRun:
     model, dv, the window image, the DP (DM) snapshot area and the shot's taps."""
 
     def __init__(self, physics, window, spacing, dt, shot, model, dmodel, nsponge, freesurface, freqQ, qMin,
-                 qInterior, snapshots):
+                 qInterior, snapshots, codec=None, learned=True):
         self.physics, self.window, self.shot = physics, window, shot
         self.nf = nf = 1 if physics == "iso" else 2
         nz, ny, nx = window.size
@@ -104,12 +170,10 @@ class _ShotRun:
         self.stream = stream = prop._lib.GetStream(prop.p)
         for name in MODEL_FIELDS[physics]:
             ginsu.extract(model[name], window, prop3d.device_ptr(prop, name), stream)
-        self.own_snap = snapshots is None
-        if self.own_snap:
-            snapshots = ginsu.DeviceBuffer(4 * ncell * nt * nf)
-        if snapshots.nbytes < 4 * ncell * nt * nf:
-            raise ValueError("snapshot buffer too small")
-        self.snapshots = snapshots
+        if codec is None:
+            self.store = _ExactStore(ncell, nt, nf, snapshots)
+        else:
+            self.store = _CodecStore(window.size, nt, nf, codec, learned)
         self.dv = ginsu.DeviceBuffer(4 * ncell)
         ginsu.extract(dmodel, window, self.dv.ptr, stream)
         self.gimg = ginsu.DeviceBuffer(4 * ncell)
@@ -124,11 +188,18 @@ class _ShotRun:
                                               fs_index=fsi)
         self.born_data = None
 
-    def snap(self, kind, n):  # address of DP_n / DM_n, n 1-based
-        return self.snapshots.ptr + 4 * self.ncell * ((n - 1) * self.nf + kind)
+    def snap(self, kind, n):  # where the step writes DP_n / DM_n (n 1-based)
+        return self.store.out(kind, n)
 
-    def dps(self, n):
-        return (self.snap(0, n),) if self.nf == 1 else (self.snap(0, n), self.snap(1, n))
+    def dps(self, n):  # where the steps write DP_n (DM_n)
+        return tuple(self.store.out(k, n) for k in range(self.nf))
+
+    def commit(self, n):  # after the step that wrote DP_n (DM_n)
+        for k in range(self.nf):
+            self.store.commit(k, n, self.stream)
+
+    def dps_in(self, n):  # DP_n (DM_n) for reading
+        return tuple(self.store.get(k, n, self.stream) for k in range(self.nf))
 
     def zero_fields(self):
         wf = ("pCur", "pOld") + (("mCur", "mOld") if self.nf == 2 else ())
@@ -170,8 +241,7 @@ class _ShotRun:
     def free(self):
         self.dv.free()
         self.gimg.free()
-        if self.own_snap:
-            self.snapshots.free()
+        self.store.free()
         prop3d.free(self.prop)
 
 
@@ -200,6 +270,7 @@ def _run_coupled(runs):
         for r in runs:
             lib, p = r.lib, r.prop.p
             lib.TimeStepBornCoupled_V(p, r.dv.ptr, r.snap(0, it))
+            r.commit(it)
             lib.InjectRegistered(p, it)
             lib.ExtractRegistered(p, it)
     for r in runs:
@@ -223,12 +294,14 @@ def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
         if batched:
             _batch_step(runs, 1, list(zip(*[r.dps(it) for r in runs])))
             for r in runs:
+                r.commit(it)
                 r.lib.InjectRegistered(r.prop.p, it)
             continue
         for r in runs:
             lib, p = r.lib, r.prop.p
             if fused:
                 lib.TimeStepSaveDP(p, *r.dps(it))
+                r.commit(it)
                 lib.InjectRegistered(p, it)
             else:
                 lib.TimeStepInjectExtract(p, it)
@@ -236,6 +309,7 @@ def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
                 ginsu.copy(r.snap(0, it), prop3d.device_ptr(r.prop, "pSpace"), 4 * r.ncell, r.stream)
                 if r.nf == 2:
                     ginsu.copy(r.snap(1, it), prop3d.device_ptr(r.prop, "mSpace"), 4 * r.ncell, r.stream)
+                r.commit(it)
     for r in runs:
         r.prop._check()
     # 2. Born forward: d = J dv
@@ -243,15 +317,15 @@ def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
         r.setup_born()
     for it in range(1, (0 if coupled else nt) + 1):
         if batched:
-            _batch_step(runs, 2, [[r.dv.ptr for r in runs]] + list(zip(*[r.dps(it) for r in runs])))
+            _batch_step(runs, 2, [[r.dv.ptr for r in runs]] + list(zip(*[r.dps_in(it) for r in runs])))
         for r in runs:
             lib, p = r.lib, r.prop.p
             if not batched:
                 if fused:
-                    lib.TimeStepBornInjection_V(p, r.dv.ptr, *r.dps(it))
+                    lib.TimeStepBornInjection_V(p, r.dv.ptr, *r.dps_in(it))
                 else:
                     lib.TimeStep(p)
-                    lib.ForwardBornInjection_V(p, r.dv.ptr, *r.dps(it))
+                    lib.ForwardBornInjection_V(p, r.dv.ptr, *r.dps_in(it))
             lib.ExtractRegistered(p, it)
     for r in runs:
         r.prop._check()
@@ -265,7 +339,7 @@ def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
             if m == 1:
                 _batch_step(runs, 0, [])
             else:
-                _batch_step(runs, 3, [[r.gimg.ptr for r in runs]] + list(zip(*[r.dps(n) for r in runs])))
+                _batch_step(runs, 3, [[r.gimg.ptr for r in runs]] + list(zip(*[r.dps_in(n) for r in runs])))
             if m <= nt:
                 for r in runs:
                     r.lib.InjectRegistered(r.prop.p, m)
@@ -273,20 +347,20 @@ def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
         for r in runs:
             lib, p = r.lib, r.prop.p
             if m >= 2 and fused:  # the image reads pOld after the step: the injection below does not touch it
-                lib.TimeStepAdjointBornAccumulation_V(p, r.gimg.ptr, *r.dps(n))
+                lib.TimeStepAdjointBornAccumulation_V(p, r.gimg.ptr, *r.dps_in(n))
             else:
                 lib.TimeStep(p)
             if m <= nt:
                 lib.InjectRegistered(p, m)
             if m >= 2 and not fused:
-                lib.AdjointBornAccumulation_V(p, r.gimg.ptr, *r.dps(n))
+                lib.AdjointBornAccumulation_V(p, r.gimg.ptr, *r.dps_in(n))
     for r in runs:
         r.prop._check()
 
 
 def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=20, freesurface=True, freqQ=5.0,
               qMin=0.1, qInterior=100.0, snapshots=None, adjoint_data=None, keep_born_data=True, fused=True,
-              coupled=None):
+              coupled=None, snapshot_codec=None, learned=True):
     """Run the three passes of one shot on `window` and add its image into `image`.
 
     model: {'v','b'[,'eps','eta','f']} -> device address of the big grid's field; dmodel: device
@@ -296,15 +370,17 @@ def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=
     data of pass 2, i.e. image = J^T J dv).  fused: each pass's per-step pair as one fused step
     (TimeStepSaveDP, TimeStepBornInjection_V, TimeStepAdjointBornAccumulation_V; the same
     results as the separate calls).  coupled (default: Iso with fused): passes 1 and 2 as one
-    pass over both wavefields (TimeStepBornCoupled_V; the same results again).  Returns a dict
-    with the Born data and timings."""
+    pass over both wavefields (TimeStepBornCoupled_V; the same results again).  snapshot_codec: a
+    ``lcodec.LearnedCodec`` to keep DP (DM) as learned-codec codes instead of exactly (decoded
+    by its network, or by the plain dequantisation with learned=False).  Returns a dict with the
+    Born data and timings."""
     if physics not in _CLS:
         raise ValueError("physics must be 'iso' or 'vti'")
     if coupled is None:
         coupled = fused and physics == "iso"
     t0 = time.perf_counter()
     run = _ShotRun(physics, window, spacing, dt, shot, model, dmodel, nsponge, freesurface, freqQ, qMin, qInterior,
-                   snapshots)
+                   snapshots, snapshot_codec, learned)
     t1 = time.perf_counter()
     try:
         _run_passes([run], fused, None if adjoint_data is None else [adjoint_data], keep_born_data, coupled)

@@ -23,6 +23,8 @@ LIBS = {
     "libwavefd_codec.so": ("codec.hip", []),
     # ginsu windows, shared image and device helpers of the Born driver; additive
     "libwavefd_ginsu.so": ("ginsu.hip", []),
+    # learned snapshot codec (quantiser + convolutional decoder on the matrix cores); additive
+    "libwavefd_lcodec.so": ("lcodec.hip", []),
 }
 
 ARCH = os.environ.get("WAVEFD_ARCH", "gfx950")

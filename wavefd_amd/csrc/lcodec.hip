@@ -1,0 +1,855 @@
+// libwavefd_lcodec -- learned snapshot codec for the Born/RTM gradient's forward snapshots.
+//
+// The reference stores snapshots exactly or through CvxCompress (src/compressedio.jl:1-148;
+// CvxCompress is un-vendored).  This codec has two parts:
+//   * encoder (fixed): 8^3 blocks (cut at the grid's ends), one scale per block -- the block's
+//     max |x| rounded up to bfloat16 -- and 11 levels per value, q = rint(5 x / s) in -5..5,
+//     packed 9 base-11 digits per 32-bit word (11^9 < 2^32): 57 words per block, i.e. 3.59 bits
+//     per value with the scale and the block's choice bit, 8.9x smaller than float32;
+//   * decoder (learned): the dequantised field q_hat = s (d - 5) / 5 is corrected by a small
+//     3D convolutional network, out = q_hat + g f(q_hat / g) (g: the largest block scale, kept
+//     in the code's header), f(q) = (F(q) - F(-q)) / 2 with F = conv3 . relu . conv2 . relu .
+//     conv1, each conv 3x3x3 without bias, 16 channels (conv1 reads one channel, conv3 writes
+//     one), trained on snapshots the propagator generates (MSE against the exact snapshot).
+//     f is odd and positively homogeneous, like the wave equation's response to its source and
+//     like the quantiser: f(0) = 0 and f(-q) = -f(q), so the decoder adds no offset and no
+//     rectified part, errors that would add up coherently in the imaging condition.  The
+//     correction is clamped to half a quantisation step, |out - q_hat| <= s / 10: the exact
+//     value lies in that cell, so the decode never leaves it (a consistent reconstruction).
+//   * encoder's choice: the encoder, which knows the exact field, runs the decoder and keeps
+//     one bit per block saying whether the learned or the plain decode is closer there; the
+//     learned decode of a block is then never worse than the plain one (the network, trained on
+//     typical blocks, would flatten the sharp peaks of the source region).  1 bit per 512
+//     values.
+// The network's convolutions, forward and backward (data and weights), run on the matrix
+// cores: v_mfma_f32_16x16x16_bf16 with bfloat16 weights and activations and float32
+// accumulation.  Every conv is computed as 16 -> 16 channels with zero-padded weights, so one
+// kernel serves all three layers and their transposes.
+//
+// Layouts: a volume batch is npatch volumes of X x Y x Z voxels, z fastest (the propagators'
+// x-slowest layout with X = nx, Y = ny, Z = nz).  Activations and activation gradients are
+// bfloat16, 16 channels per voxel, channel fastest.  Weights W[t][co][c] (t = tap, 27), with
+// the matrix-core B operand layout Wb[t][n][k] (n output, k input channel).
+//
+// Matrix-core operand layout of v_mfma_f32_16x16x16_bf16 (wave64): lane l supplies
+// A[m = l % 16][k = 4 (l / 16) + j] and B[k = 4 (l / 16) + j][n = l % 16] for j = 0..3, and
+// holds D[m = 4 (l / 16) + j][n = l % 16].
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+namespace {
+
+std::string g_err;
+
+#define LC_TRY(x)                                                        \
+    do {                                                                 \
+        hipError_t e_ = (x);                                             \
+        if (e_ != hipSuccess) {                                          \
+            g_err = std::string(#x) + ": " + hipGetErrorString(e_);      \
+            return -1;                                                   \
+        }                                                                \
+    } while (0)
+
+constexpr uint32_t MAGIC = 0x5746444cu;  // "WFDL"
+constexpr int BS = 8, BV = BS * BS * BS;  // block extent and values per block
+constexpr int WPB = 57;                   // payload words per block (9 digits per word)
+constexpr int LV = 5;                     // levels -LV..LV
+constexpr int CH = 16;                    // channels
+constexpr int NT = 27;                    // taps
+constexpr int LAYER_W = NT * CH * CH;     // weights per layer (padded)
+constexpr int LAYER_P = LAYER_W + CH;     // weights + bias per layer
+constexpr int NPARAM = 3 * LAYER_P;
+
+// ---- bfloat16 ---------------------------------------------------------------------------
+__host__ __device__ inline uint16_t f2bf(float f) {  // round to nearest even
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+__host__ __device__ inline uint16_t f2bf_up(float f) {  // smallest bfloat16 >= f (f >= 0, finite)
+    uint16_t b = f2bf(f);
+    if (bf2f(b) < f) b = (uint16_t)(b + 1);
+    return b;
+}
+
+// ---- quantiser --------------------------------------------------------------------------
+struct QGeo {
+    long n1, n2, n3, m1, m2, m3;
+    __host__ __device__ long nblocks() const { return m1 * m2 * m3; }
+};
+inline QGeo qgeo(long n1, long n2, long n3) {
+    return {n1, n2, n3, (n1 + BS - 1) / BS, (n2 + BS - 1) / BS, (n3 + BS - 1) / BS};
+}
+__host__ __device__ inline long scale_words(const QGeo &g) { return (g.nblocks() + 1) / 2; }
+// one bit per block after the scales: 1 = the block's decode is the learned one
+__host__ __device__ inline long flag_off(const QGeo &g) { return 8 + scale_words(g); }
+__host__ __device__ inline long pay_off(const QGeo &g) { return flag_off(g) + (g.nblocks() + 31) / 32; }
+inline long code_words(const QGeo &g) { return pay_off(g) + WPB * g.nblocks(); }
+
+__device__ __forceinline__ long block_elem(const QGeo &g, long blk, int e, bool &in) {
+    const long k1 = blk % g.m1, k2 = (blk / g.m1) % g.m2, k3 = blk / (g.m1 * g.m2);
+    const long i1 = k1 * BS + (e & 7), i2 = k2 * BS + ((e >> 3) & 7), i3 = k3 * BS + (e >> 6);
+    in = i1 < g.n1 && i2 < g.n2 && i3 < g.n3;
+    return in ? (i3 * g.n2 + i2) * g.n1 + i1 : 0;
+}
+
+__global__ void __launch_bounds__(256) k_encode(QGeo g, const float *__restrict__ x, uint32_t *__restrict__ code) {
+    const long blk = blockIdx.x;
+    __shared__ float red[256];
+    __shared__ float sv[BV];
+    float m = 0.0f;
+    for (int e = threadIdx.x; e < BV; e += 256) {
+        bool in;
+        const long k = block_elem(g, blk, e, in);
+        const float v = in ? x[k] : 0.0f;
+        sv[e] = v;
+        m = fmaxf(m, fabsf(v));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const uint16_t sb = f2bf_up(red[0]);
+    const float s = bf2f(sb);
+    uint16_t *scales = reinterpret_cast<uint16_t *>(code + 8);
+    if (threadIdx.x == 0) {
+        scales[blk] = sb;
+        // the largest block scale (non-negative floats order as their bit patterns)
+        atomicMax(code + 7, (uint32_t)sb << 16);
+    }
+    uint32_t *pay = code + pay_off(g) + WPB * blk;
+    if (threadIdx.x < WPB) {
+        uint32_t w = 0, p = 1;
+        for (int j = 0; j < 9; j++) {
+            const int e = threadIdx.x * 9 + j;
+            int q = 0;
+            if (e < BV && s > 0.0f) {
+                q = (int)rintf(sv[e] / s * (float)LV);
+                q = q < -LV ? -LV : (q > LV ? LV : q);
+            }
+            w += (uint32_t)(q + LV) * p;
+            p *= 11u;
+        }
+        pay[threadIdx.x] = w;
+    }
+}
+
+__device__ __forceinline__ float dequant(float s, int d) { return s * (float)(d - LV) / (float)LV; }
+
+// half a quantisation step of every value's block, s / 10 (the bound of |x - x_hat|)
+__global__ void __launch_bounds__(256) k_halfstep(QGeo g, const uint32_t *__restrict__ code, float *__restrict__ out) {
+    const long blk = blockIdx.x;
+    const uint16_t *scales = reinterpret_cast<const uint16_t *>(code + 8);
+    const float h = bf2f(scales[blk]) / (float)(2 * LV);
+    for (int e = threadIdx.x; e < BV; e += blockDim.x) {
+        bool in;
+        const long k = block_elem(g, blk, e, in);
+        if (in) out[k] = h;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_dequantize(QGeo g, const uint32_t *__restrict__ code, float *__restrict__ out) {
+    const long blk = blockIdx.x;
+    const uint16_t *scales = reinterpret_cast<const uint16_t *>(code + 8);
+    const float s = bf2f(scales[blk]);
+    const uint32_t *pay = code + pay_off(g) + WPB * blk;
+    if (threadIdx.x < WPB) {
+        uint32_t w = pay[threadIdx.x];
+        for (int j = 0; j < 9; j++) {
+            const int e = threadIdx.x * 9 + j;
+            const int d = (int)(w % 11u);
+            w /= 11u;
+            if (e < BV) {
+                bool in;
+                const long k = block_elem(g, blk, e, in);
+                if (in) out[k] = dequant(s, d);
+            }
+        }
+    }
+}
+
+// encoder's choice per block: flag the block when the learned decode o is closer to the exact x
+// than the plain dequantisation qh (sums of squares over the block's values)
+__global__ void __launch_bounds__(256) k_select(QGeo g, const float *__restrict__ x, const float *__restrict__ qh,
+                                                const float *__restrict__ o, uint32_t *__restrict__ code) {
+    const long blk = blockIdx.x;
+    __shared__ float rq[256], ro[256];
+    float eq = 0.0f, eo = 0.0f;
+    for (int e = threadIdx.x; e < BV; e += 256) {
+        bool in;
+        const long k = block_elem(g, blk, e, in);
+        if (in) {
+            const float a = qh[k] - x[k], b = o[k] - x[k];
+            eq += a * a;
+            eo += b * b;
+        }
+    }
+    rq[threadIdx.x] = eq;
+    ro[threadIdx.x] = eo;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            rq[threadIdx.x] += rq[threadIdx.x + st];
+            ro[threadIdx.x] += ro[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && ro[0] < rq[0]) atomicOr(code + flag_off(g) + (blk >> 5), 1u << (blk & 31));
+}
+
+// decoder's side: blocks without the flag take the plain dequantisation
+__global__ void __launch_bounds__(256) k_apply_flags(QGeo g, const uint32_t *__restrict__ code,
+                                                     const float *__restrict__ qh, float *__restrict__ out) {
+    const long blk = blockIdx.x;
+    if ((code[flag_off(g) + (blk >> 5)] >> (blk & 31)) & 1u) return;
+    for (int e = threadIdx.x; e < BV; e += 256) {
+        bool in;
+        const long k = block_elem(g, blk, e, in);
+        if (in) out[k] = qh[k];
+    }
+}
+
+// ---- convolutions on the matrix cores --------------------------------------------------
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+struct Vol {
+    long npatch, X, Y, Z;
+    __host__ __device__ long nvox() const { return npatch * X * Y * Z; }
+    __host__ __device__ long ntiles() const { return npatch * X * Y * ((Z + 15) / 16); }
+};
+
+// voxel (p, x, y, z) shifted by tap t (dx, dy, dz in -1..1); -1 outside the volume
+__device__ __forceinline__ long shifted(const Vol &v, long p, long x, long y, long z, int t) {
+    const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+    const long xx = x + dx, yy = y + dy, zz = z + dz;
+    if (xx < 0 || xx >= v.X || yy < 0 || yy >= v.Y || zz < 0 || zz >= v.Z) return -1;
+    return ((p * v.X + xx) * v.Y + yy) * v.Z + zz;
+}
+
+__device__ __forceinline__ s4 ld4(const uint16_t *a) {  // 4 consecutive bfloat16 (8-byte aligned)
+    const uint2 u = *reinterpret_cast<const uint2 *>(a);
+    s4 r;
+    r.x = (short)(u.x & 0xffff);
+    r.y = (short)(u.x >> 16);
+    r.z = (short)(u.y & 0xffff);
+    r.w = (short)(u.y >> 16);
+    return r;
+}
+
+enum ConvOut { OUT_RELU = 0, OUT_RAW = 1, OUT_MASK = 2 };
+
+// out[v][n] = sum_t sum_k in[v + t][k] Wb[t][n][k] (+ epilogue); one wave per 16-voxel z-tile.
+//   OUT_RELU:     act[v][n] = bf16(relu(acc + bias[n]))
+//   OUT_RAW:      res[v] = acc + bias[0], channel 0 only, float32
+//   OUT_MASK:     act[v][n] = bf16(acc) where mask[v][n] > 0, else 0 (relu backward)
+template <int MODE>
+__global__ void __launch_bounds__(256) k_conv(Vol vol, const uint16_t *__restrict__ in, const uint16_t *__restrict__ wb,
+                                              const float *__restrict__ bias, const uint16_t *__restrict__ mask,
+                                              const float *__restrict__ base, float norm, uint16_t *__restrict__ act,
+                                              float *__restrict__ res) {
+    const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= vol.ntiles()) return;
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const long zt = (vol.Z + 15) / 16;
+    const long row = tile / zt, z0 = (tile - row * zt) * 16;
+    const long y = row % vol.Y, x = (row / vol.Y) % vol.X, p = row / (vol.Y * vol.X);
+    const long za = z0 + r;  // this lane's A-row voxel
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const s4 zero = {0, 0, 0, 0};
+    for (int t = 0; t < NT; t++) {
+        const long s = za < vol.Z ? shifted(vol, p, x, y, za, t) : -1;
+        const s4 a = s >= 0 ? ld4(in + s * CH + 4 * g) : zero;
+        const s4 b = ld4(wb + (t * CH + r) * CH + 4 * g);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    const int n = r;  // D column
+    for (int j = 0; j < 4; j++) {
+        const long z = z0 + 4 * g + j;
+        if (z >= vol.Z) continue;
+        const long v = ((p * vol.X + x) * vol.Y + y) * vol.Z + z;
+        const float a = acc[j];
+        if constexpr (MODE == OUT_RELU) {
+            const float h = a + bias[n];
+            act[v * CH + n] = f2bf(h > 0.0f ? h : 0.0f);
+        } else if constexpr (MODE == OUT_RAW) {
+            if (n == 0) res[v] = a + bias[0];
+        } else {
+            act[v * CH + n] = bf2f(mask[v * CH + n]) > 0.0f ? f2bf(a) : (uint16_t)0;
+        }
+    }
+}
+
+// partial weight gradients: G[t][co][c] = sum_v dO[v][co] in[v + t][c] over the tiles of one
+// workgroup (blockIdx.x) for tap blockIdx.y; each wave takes every 4th tile of the range and
+// writes its own partial, part[(blk * 4 + wave) * 27 + t][co][c]
+__global__ void __launch_bounds__(256) k_wgrad(Vol vol, const uint16_t *__restrict__ dO, const uint16_t *__restrict__ in,
+                                               long tiles_per_blk, float *__restrict__ part) {
+    const int t = blockIdx.y, wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const long zt = (vol.Z + 15) / 16, nt = vol.ntiles();
+    const long t0 = (long)blockIdx.x * tiles_per_blk;
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (long tile = t0 + wave; tile < t0 + tiles_per_blk && tile < nt; tile += 4) {
+        const long row = tile / zt, z0 = (tile - row * zt) * 16;
+        const long y = row % vol.Y, x = (row / vol.Y) % vol.X, p = row / (vol.Y * vol.X);
+        s4 a, b;
+        short av[4], bv[4];
+        for (int j = 0; j < 4; j++) {
+            const long z = z0 + 4 * g + j;  // K index = voxel
+            if (z < vol.Z) {
+                const long v = ((p * vol.X + x) * vol.Y + y) * vol.Z + z;
+                av[j] = (short)dO[v * CH + r];  // A[m = co = r][k = voxel]
+                const long s = shifted(vol, p, x, y, z, t);
+                bv[j] = s >= 0 ? (short)in[s * CH + r] : (short)0;  // B[k = voxel][n = c = r]
+            } else {
+                av[j] = 0;
+                bv[j] = 0;
+            }
+        }
+        a.x = av[0]; a.y = av[1]; a.z = av[2]; a.w = av[3];
+        b.x = bv[0]; b.y = bv[1]; b.z = bv[2]; b.w = bv[3];
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    float *o = part + ((long)(blockIdx.x * 4 + wave) * NT + t) * CH * CH;
+    for (int j = 0; j < 4; j++) o[(4 * g + j) * CH + r] = acc[j];  // D[m = co][n = c]
+}
+
+// partial bias gradients: part[blk][co] = sum_v dO[v][co] over the workgroup's voxel range
+__global__ void __launch_bounds__(256) k_bgrad(long nvox, const uint16_t *__restrict__ dO, long vox_per_blk,
+                                               float *__restrict__ part) {
+    __shared__ float red[256];
+    const int co = threadIdx.x & 15, sub = threadIdx.x >> 4;  // 16 sub-lanes per channel
+    const long v0 = (long)blockIdx.x * vox_per_blk;
+    float s = 0.0f;
+    for (long v = v0 + sub; v < v0 + vox_per_blk && v < nvox; v += 16) s += bf2f(dO[v * CH + co]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        float a = 0.0f;
+        for (int k = 0; k < 16; k++) a += red[k * 16 + threadIdx.x];
+        part[blockIdx.x * CH + threadIdx.x] = a;
+    }
+}
+
+// sum of nparts partial vectors of length len, in part order: out[i] = sum_p part[p][i]
+__global__ void k_reduce_parts(const float *__restrict__ part, long nparts, long len, float *__restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    float s = 0.0f;
+    for (long p = 0; p < nparts; p++) s += part[p * len + i];
+    out[i] = s;
+}
+
+// one-channel float32 field, times inv_norm -> 16-channel bfloat16 activations (channel 0) of
+// the volumes 0..n-1, and of its negation for the volumes n..2n-1
+__global__ void k_pack1(long n, const float *__restrict__ f, float inv_norm, uint16_t *__restrict__ act) {
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint16_t a = f2bf(f[v] * inv_norm), b = f2bf(-(f[v] * inv_norm));
+    for (int c = 0; c < CH; c++) {
+        act[v * CH + c] = c == 0 ? a : (uint16_t)0;
+        act[(n + v) * CH + c] = c == 0 ? b : (uint16_t)0;
+    }
+}
+
+// the correction norm (F(q) - F(-q)) / 2 from the raw outputs F of both halves, clamped to
+// +-hs[v] when hs is given
+__device__ __forceinline__ float correction(long n, long v, const float *raw, float norm, const float *hs,
+                                            bool &clamped) {
+    const float c = norm * (0.5f * (raw[v] - raw[n + v]));
+    clamped = false;
+    if (hs) {
+        const float h = hs[v];
+        if (c > h || c < -h) {
+            clamped = true;
+            return c > h ? h : -h;
+        }
+    }
+    return c;
+}
+
+// out = q + correction
+__global__ void k_combine(long n, const float *__restrict__ q, const float *__restrict__ raw, float norm,
+                          const float *hs, float *out) {  // hs may be out
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    bool cl;
+    out[v] = q[v] + correction(n, v, raw, norm, hs, cl);
+}
+
+// MSE loss gradient in the raw outputs of both halves, +-norm/2 x 2 (out - target) / n, into
+// channel 0 of 16-channel bfloat16 (volumes 0..n-1 and n..2n-1), and the per-workgroup partial
+// sums of (out - target)^2
+__global__ void __launch_bounds__(256) k_mse(long n, const float *__restrict__ out, const float *__restrict__ target,
+                                             const float *__restrict__ raw, float norm, const float *__restrict__ hs,
+                                             uint16_t *__restrict__ g, float *__restrict__ part) {
+    __shared__ float red[256];
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    float e2 = 0.0f;
+    if (v < n) {
+        const float e = out[v] - target[v];
+        e2 = e * e;
+        bool cl;
+        (void)correction(n, v, raw, norm, hs, cl);
+        const float d = cl ? 0.0f : 0.5f * norm * (2.0f * e / (float)n);  // no gradient through the clamp
+        for (int c = 0; c < CH; c++) {
+            g[v * CH + c] = c == 0 ? f2bf(d) : (uint16_t)0;
+            g[(n + v) * CH + c] = c == 0 ? f2bf(-d) : (uint16_t)0;
+        }
+    }
+    red[threadIdx.x] = e2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// weights of one layer (float32 W[t][co][c]) -> matrix-core B operands: forward Wb[t][co][c],
+// transposed Wt[t'][c][co] = W[26 - t'][co][c] (the data gradient's conv)
+__global__ void k_prep_weights(const float *__restrict__ w, uint16_t *__restrict__ wb, uint16_t *__restrict__ wt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= LAYER_W) return;
+    const int t = i / (CH * CH), co = (i / CH) % CH, c = i % CH;
+    const uint16_t b = f2bf(w[i]);
+    wb[i] = b;
+    wt[((NT - 1 - t) * CH + c) * CH + co] = b;
+}
+
+__global__ void k_apply_mask(long n, float *__restrict__ g, const float *__restrict__ mask) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) g[i] *= mask[i];
+}
+
+// Adam on the packed parameters; entries with mask 0 (the zero padding) stay 0
+__global__ void k_adam(long n, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, const float *__restrict__ mask, float lr, float b1, float b2, float c1,
+                       float c2, float eps) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] * mask[i];
+    m[i] = b1 * m[i] + (1.0f - b1) * gi;
+    v[i] = b2 * v[i] + (1.0f - b2) * gi * gi;
+    p[i] = (p[i] - lr * (m[i] / c1) / (sqrtf(v[i] / c2) + eps)) * mask[i];
+}
+
+// patches: dst[i] = scale x the src cube of P^3 at origins (o1, o2, o3)[i] (fast .. slow) of an
+// n1 x n2 x n3 field (z fastest); output volume i is X = Y = Z = P with z fastest
+__global__ void k_gather(const float *__restrict__ src, long n1, long n2, long n3, const long *__restrict__ org,
+                         long P, long npatch, float scale, float *__restrict__ dst) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = P * P * P;
+    if (i >= per * npatch) return;
+    const long pi = i / per, e = i - pi * per;
+    const long z = e % P, y = (e / P) % P, x = e / (P * P);
+    const long o1 = org[3 * pi], o2 = org[3 * pi + 1], o3 = org[3 * pi + 2];
+    dst[i] = src[((o3 + x) * n2 + (o2 + y)) * n1 + (o1 + z)] * scale;
+}
+
+// ---- the network --------------------------------------------------------------------------
+struct Net {
+    float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *mask = nullptr;  // NPARAM each
+    uint16_t *wb = nullptr, *wt = nullptr;  // 3 layers of B operands each
+    long step = 0;
+    long cap = 0;  // voxels the workspace holds
+    uint16_t *x0 = nullptr, *h1 = nullptr, *h2 = nullptr, *d = nullptr, *d2 = nullptr;
+    float *part = nullptr;
+    long part_cap = 0;
+    float *lossp = nullptr;
+    float *raw = nullptr;  // conv3's raw output of both halves
+    ~Net() {
+        for (void *q : {(void *)p, (void *)g, (void *)m, (void *)v, (void *)mask, (void *)wb, (void *)wt, (void *)x0,
+                        (void *)h1, (void *)h2, (void *)d, (void *)d2, (void *)part, (void *)lossp, (void *)raw})
+            if (q) (void)hipFree(q);
+    }
+};
+
+float *layer_w(float *p, int l) { return p + (long)l * LAYER_P; }
+float *layer_b(float *p, int l) { return p + (long)l * LAYER_P + LAYER_W; }
+
+// workspace for nvox voxels of input, i.e. 2 nvox of activations (the field and its negation)
+long ensure_ws(Net *n, long nvox) {
+    if (nvox <= n->cap) return 0;
+    for (uint16_t **q : {&n->x0, &n->h1, &n->h2, &n->d, &n->d2}) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+        LC_TRY(hipMalloc(q, sizeof(uint16_t) * CH * 2 * nvox));
+    }
+    if (n->raw) (void)hipFree(n->raw);
+    n->raw = nullptr;
+    LC_TRY(hipMalloc(&n->raw, sizeof(float) * 2 * nvox));
+    n->cap = nvox;
+    return 0;
+}
+
+long ensure_part(Net *n, long len) {
+    if (len <= n->part_cap) return 0;
+    if (n->part) (void)hipFree(n->part);
+    n->part = nullptr;
+    LC_TRY(hipMalloc(&n->part, sizeof(float) * len));
+    n->part_cap = len;
+    return 0;
+}
+
+long prep(Net *n, hipStream_t s) {
+    for (int l = 0; l < 3; l++) {
+        k_prep_weights<<<(LAYER_W + 255) / 256, 256, 0, s>>>(layer_w(n->p, l), n->wb + (long)l * LAYER_W,
+                                                            n->wt + (long)l * LAYER_W);
+    }
+    LC_TRY(hipGetLastError());
+    return 0;
+}
+
+unsigned grid_tiles(const Vol &v) { return (unsigned)((v.ntiles() + 3) / 4); }
+
+// both halves: the volumes and their negations
+Vol twice(const Vol &v) { return Vol{2 * v.npatch, v.X, v.Y, v.Z}; }
+
+// out = q + norm f(q / norm) for the volumes q (float32); keeps x0, h1, h2 (of both halves) for
+// the backward pass
+long forward(Net *n, const Vol &vol, const float *q, float *out, float norm, const float *hs, hipStream_t s) {
+    const long nv = vol.nvox();
+    const Vol v2 = twice(vol);
+    if (ensure_ws(n, nv)) return -1;
+    if (prep(n, s)) return -1;
+    k_pack1<<<(unsigned)((nv + 255) / 256), 256, 0, s>>>(nv, q, 1.0f / norm, n->x0);
+    k_conv<OUT_RELU><<<grid_tiles(v2), 256, 0, s>>>(v2, n->x0, n->wb, layer_b(n->p, 0), nullptr, nullptr, 0.0f,
+                                                    n->h1, nullptr);
+    k_conv<OUT_RELU><<<grid_tiles(v2), 256, 0, s>>>(v2, n->h1, n->wb + LAYER_W, layer_b(n->p, 1), nullptr, nullptr,
+                                                    0.0f, n->h2, nullptr);
+    k_conv<OUT_RAW><<<grid_tiles(v2), 256, 0, s>>>(v2, n->h2, n->wb + 2 * LAYER_W, layer_b(n->p, 2), nullptr,
+                                                   nullptr, 0.0f, nullptr, n->raw);
+    k_combine<<<(unsigned)((nv + 255) / 256), 256, 0, s>>>(nv, q, n->raw, norm, hs, out);
+    LC_TRY(hipGetLastError());
+    return 0;
+}
+
+// weight and bias gradients of one layer from its output gradient dO and its input
+long layer_grads(Net *n, const Vol &vol, const uint16_t *dO, const uint16_t *in, int l, hipStream_t s) {
+    const long nt = vol.ntiles(), nv = vol.nvox();
+    const long tpb = 256;  // tiles per workgroup
+    const long nblk = (nt + tpb - 1) / tpb;
+    const long vpb = 4096;
+    const long nbb = (nv + vpb - 1) / vpb;
+    if (ensure_part(n, std::max(nblk * 4 * NT * CH * CH, nbb * CH))) return -1;
+    k_wgrad<<<dim3((unsigned)nblk, NT), 256, 0, s>>>(vol, dO, in, tpb, n->part);
+    k_reduce_parts<<<(LAYER_W + 255) / 256, 256, 0, s>>>(n->part, nblk * 4, LAYER_W, layer_w(n->g, l));
+    k_bgrad<<<(unsigned)nbb, 256, 0, s>>>(nv, dO, vpb, n->part);
+    k_reduce_parts<<<1, CH, 0, s>>>(n->part, nbb, CH, layer_b(n->g, l));
+    LC_TRY(hipGetLastError());
+    return 0;
+}
+
+long loss_grad(Net *n, const Vol &vol, const float *q, const float *target, float *out, float norm, const float *hs,
+               float *loss, hipStream_t s) {
+    const long nv = vol.nvox();
+    if (forward(n, vol, q, out, norm, hs, s)) return -1;
+    const long nb = (nv + 255) / 256;
+    if (ensure_part(n, nb)) return -1;
+    k_mse<<<(unsigned)nb, 256, 0, s>>>(nv, out, target, n->raw, norm, hs, n->d, n->part);
+    k_reduce_parts<<<1, 1, 0, s>>>(n->part, nb, 1, n->lossp);
+    // layer 3: dO = d (channel 0), over both halves
+    const Vol v2 = twice(vol);
+    if (layer_grads(n, v2, n->d, n->h2, 2, s)) return -1;
+    k_conv<OUT_MASK><<<grid_tiles(v2), 256, 0, s>>>(v2, n->d, n->wt + 2 * LAYER_W, nullptr, n->h2, nullptr, 0.0f,
+                                                    n->d2, nullptr);
+    if (layer_grads(n, v2, n->d2, n->h1, 1, s)) return -1;
+    k_conv<OUT_MASK><<<grid_tiles(v2), 256, 0, s>>>(v2, n->d2, n->wt + LAYER_W, nullptr, n->h1, nullptr, 0.0f,
+                                                    n->d, nullptr);
+    if (layer_grads(n, v2, n->d, n->x0, 0, s)) return -1;
+    k_apply_mask<<<(NPARAM + 255) / 256, 256, 0, s>>>(NPARAM, n->g, n->mask);  // no padding, no biases
+    LC_TRY(hipGetLastError());
+    if (loss) {
+        float l = 0.0f;
+        LC_TRY(hipMemcpyAsync(&l, n->lossp, sizeof(float), hipMemcpyDeviceToHost, s));
+        LC_TRY(hipStreamSynchronize(s));
+        *loss = l / (float)nv;
+    }
+    return 0;
+}
+
+void host_mask(std::vector<float> &mk) {
+    mk.assign(NPARAM, 0.0f);
+    for (int l = 0; l < 3; l++)
+        for (int t = 0; t < NT; t++)
+            for (int co = 0; co < CH; co++)
+                for (int c = 0; c < CH; c++) {
+                    const bool ok = (l == 0 ? c == 0 : true) && (l == 2 ? co == 0 : true);
+                    mk[(long)l * LAYER_P + (t * CH + co) * CH + c] = ok ? 1.0f : 0.0f;
+                }
+    // no biases: f(0) = 0 and f(a q) = a f(q) for a > 0
+}
+
+}  // namespace
+
+#pragma GCC visibility push(default)
+extern "C" {
+
+const char *wavefd_lcodec_LastError(void) { return g_err.c_str(); }
+
+long wavefd_lcodec_code_words(long n1, long n2, long n3) {
+    if (n1 <= 0 || n2 <= 0 || n3 <= 0) return -1;
+    return code_words(qgeo(n1, n2, n3));
+}
+
+long wavefd_lcodec_encode(const float *x, long n1, long n2, long n3, uint32_t *code, void *stream) {
+    if (n1 <= 0 || n2 <= 0 || n3 <= 0 || !x || !code) {
+        g_err = "encode: bad arguments";
+        return -1;
+    }
+    const QGeo g = qgeo(n1, n2, n3);
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t hdr[8] = {MAGIC, (uint32_t)n1, (uint32_t)n2, (uint32_t)n3, (uint32_t)BS, (uint32_t)LV, 9u, 0u};
+    LC_TRY(hipMemcpyAsync(code, hdr, sizeof(hdr), hipMemcpyHostToDevice, s));
+    LC_TRY(hipMemsetAsync(code + 8, 0, sizeof(uint32_t) * (pay_off(g) - 8), s));  // scales and flags
+    k_encode<<<(unsigned)g.nblocks(), 256, 0, s>>>(g, x, code);
+    LC_TRY(hipGetLastError());
+    LC_TRY(hipStreamSynchronize(s));  // the header copy reads host memory
+    return 0;
+}
+
+// the grid dimensions stored in a code (device memory)
+static long read_header(const uint32_t *code, long *n, hipStream_t s, float *norm = nullptr) {
+    uint32_t hdr[8];
+    LC_TRY(hipMemcpyAsync(hdr, code, sizeof(hdr), hipMemcpyDeviceToHost, s));
+    LC_TRY(hipStreamSynchronize(s));
+    if (hdr[0] != MAGIC || hdr[4] != (uint32_t)BS || hdr[5] != (uint32_t)LV) {
+        g_err = "not a learned-codec code";
+        return -1;
+    }
+    n[0] = hdr[1];
+    n[1] = hdr[2];
+    n[2] = hdr[3];
+    if (norm) {
+        float g;
+        memcpy(&g, &hdr[7], 4);
+        *norm = g > 0.0f ? g : 1.0f;
+    }
+    return 0;
+}
+
+// the code's largest block scale (the decoder's normalisation; 1 for an all-zero field)
+float wavefd_lcodec_norm(const uint32_t *code) {
+    long n[3];
+    float g = 1.0f;
+    if (read_header(code, n, nullptr, &g)) return -1.0f;
+    return g;
+}
+
+long wavefd_lcodec_dequantize(const uint32_t *code, float *out, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    long n[3];
+    if (read_header(code, n, s)) return -1;
+    const QGeo g = qgeo(n[0], n[1], n[2]);
+    k_dequantize<<<(unsigned)g.nblocks(), 64, 0, s>>>(g, code, out);
+    LC_TRY(hipGetLastError());
+    return 0;
+}
+
+// half a quantisation step for every value of the field, s / 10 (the bound of |x - x_hat|)
+long wavefd_lcodec_halfstep(const uint32_t *code, float *out, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    long n[3];
+    if (read_header(code, n, s)) return -1;
+    const QGeo g = qgeo(n[0], n[1], n[2]);
+    k_halfstep<<<(unsigned)g.nblocks(), 256, 0, s>>>(g, code, out);
+    LC_TRY(hipGetLastError());
+    return 0;
+}
+
+long wavefd_lcodec_nparams(void) { return NPARAM; }
+
+void *wavefd_lcodec_net_create(void) {
+    Net *n = new Net();
+    bool ok = true;
+    for (float **q : {&n->p, &n->g, &n->m, &n->v, &n->mask}) ok = ok && hipMalloc(q, sizeof(float) * NPARAM) == hipSuccess;
+    ok = ok && hipMalloc(&n->wb, sizeof(uint16_t) * 3 * LAYER_W) == hipSuccess;
+    ok = ok && hipMalloc(&n->wt, sizeof(uint16_t) * 3 * LAYER_W) == hipSuccess;
+    ok = ok && hipMalloc(&n->lossp, sizeof(float)) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        g_err = "net_create: out of device memory";
+        delete n;
+        return nullptr;
+    }
+    std::vector<float> mk;
+    host_mask(mk);
+    std::vector<float> z(NPARAM, 0.0f);
+    ok = hipMemcpy(n->mask, mk.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice) == hipSuccess;
+    for (float *q : {n->p, n->g, n->m, n->v}) ok = ok && hipMemcpy(q, z.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        g_err = "net_create: copy failed";
+        delete n;
+        return nullptr;
+    }
+    return n;
+}
+
+void wavefd_lcodec_net_free(void *net) { delete reinterpret_cast<Net *>(net); }
+
+// parameters (host arrays of nparams floats): per layer W[t][co][c] (27 x 16 x 16) then the 16
+// biases; entries outside the real layer shapes (conv1: c > 0; conv3: co > 0) are ignored/0
+long wavefd_lcodec_net_get(void *net, float *params) {
+    Net *n = reinterpret_cast<Net *>(net);
+    LC_TRY(hipMemcpy(params, n->p, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+long wavefd_lcodec_net_set(void *net, const float *params) {
+    Net *n = reinterpret_cast<Net *>(net);
+    std::vector<float> mk, q(params, params + NPARAM);
+    host_mask(mk);
+    for (long i = 0; i < NPARAM; i++) q[i] *= mk[i];
+    LC_TRY(hipMemcpy(n->p, q.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice));
+    std::vector<float> z(NPARAM, 0.0f);
+    LC_TRY(hipMemcpy(n->m, z.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice));
+    LC_TRY(hipMemcpy(n->v, z.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice));
+    n->step = 0;
+    return 0;
+}
+
+// He-normal initial weights (fan-in 27 x real input channels); the last layer starts at zero so
+// that the untrained decoder is the plain dequantisation
+long wavefd_lcodec_net_init(void *net, unsigned long seed) {
+    std::mt19937_64 rng(seed);
+    std::vector<float> q(NPARAM, 0.0f);
+    for (int l = 0; l < 2; l++) {
+        const float sd = std::sqrt(2.0f / (float)(NT * (l == 0 ? 1 : CH)));
+        std::normal_distribution<float> nd(0.0f, sd);
+        for (int i = 0; i < LAYER_W; i++) q[(long)l * LAYER_P + i] = nd(rng);
+    }
+    return wavefd_lcodec_net_set(net, q.data());
+}
+
+// out = q + norm f(q / norm) on npatch volumes of X x Y x Z (z fastest), device arrays; the
+// correction clamped to +-hs where hs (half steps, one per voxel) is not null
+long wavefd_lcodec_forward(void *net, const float *q, long npatch, long X, long Y, long Z, float norm,
+                           const float *hs, float *out, void *stream) {
+    Net *n = reinterpret_cast<Net *>(net);
+    if (npatch <= 0 || X <= 0 || Y <= 0 || Z <= 0 || !(norm > 0.0f)) {
+        g_err = "forward: bad volume or norm";
+        return -1;
+    }
+    return forward(n, Vol{npatch, X, Y, Z}, q, out, norm, hs, (hipStream_t)stream);
+}
+
+// loss = mean (out - target)^2 and its gradient in the parameters (kept for adam; copied to
+// grad, a host array of nparams floats, when grad is not null); out receives the network output
+long wavefd_lcodec_grad(void *net, const float *q, const float *target, long npatch, long X, long Y, long Z, float norm,
+                        const float *hs, float *out, float *loss, float *grad, void *stream) {
+    Net *n = reinterpret_cast<Net *>(net);
+    if (npatch <= 0 || X <= 0 || Y <= 0 || Z <= 0 || !(norm > 0.0f)) {
+        g_err = "grad: bad volume or norm";
+        return -1;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (loss_grad(n, Vol{npatch, X, Y, Z}, q, target, out, norm, hs, loss, s)) return -1;
+    if (grad) {
+        LC_TRY(hipMemcpyAsync(grad, n->g, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, s));
+        LC_TRY(hipStreamSynchronize(s));
+    }
+    return 0;
+}
+
+long wavefd_lcodec_adam(void *net, float lr, void *stream) {
+    Net *n = reinterpret_cast<Net *>(net);
+    n->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, (float)n->step), c2 = 1.0f - std::pow(b2, (float)n->step);
+    k_adam<<<(NPARAM + 255) / 256, 256, 0, (hipStream_t)stream>>>(NPARAM, n->p, n->g, n->m, n->v, n->mask, lr, b1, b2,
+                                                                   c1, c2, 1e-8f);
+    LC_TRY(hipGetLastError());
+    return 0;
+}
+
+// the learned decode of a whole code: dequantise, then out = q_hat + g f(q_hat / g) with g the
+// code's largest block scale; scratch: a device array of the grid's size (the dequantised field)
+long wavefd_lcodec_decode(void *net, const uint32_t *code, float *scratch, float *out, void *stream) {
+    Net *n = reinterpret_cast<Net *>(net);
+    hipStream_t s = (hipStream_t)stream;
+    long d[3];
+    float norm = 1.0f;
+    if (read_header(code, d, s, &norm)) return -1;
+    if (wavefd_lcodec_dequantize(code, scratch, stream)) return -1;
+    // the half steps go to out, which the network reads before it writes its result there
+    // (k_combine reads hs[v] and writes out[v] in the same thread)
+    if (wavefd_lcodec_halfstep(code, out, stream)) return -1;
+    if (forward(n, Vol{1, d[2], d[1], d[0]}, scratch, out, norm, out, s)) return -1;
+    const QGeo g = qgeo(d[0], d[1], d[2]);
+    k_apply_flags<<<(unsigned)g.nblocks(), 256, 0, s>>>(g, code, scratch, out);
+    LC_TRY(hipGetLastError());
+    return 0;
+}
+
+// encode with the encoder's per-block choice: quantise, run the learned decode, and flag the
+// blocks where it is closer to x than the plain dequantisation (so that no block's decode is
+// worse than plain); scratch1, scratch2: device arrays of the field's size
+long wavefd_lcodec_encode_learned(void *net, const float *x, long n1, long n2, long n3, uint32_t *code, float *scratch1,
+                                  float *scratch2, void *stream) {
+    Net *n = reinterpret_cast<Net *>(net);
+    hipStream_t s = (hipStream_t)stream;
+    if (wavefd_lcodec_encode(x, n1, n2, n3, code, stream)) return -1;
+    float norm = 1.0f;
+    long d[3];
+    if (read_header(code, d, s, &norm)) return -1;
+    if (wavefd_lcodec_dequantize(code, scratch1, stream)) return -1;
+    if (wavefd_lcodec_halfstep(code, scratch2, stream)) return -1;
+    if (forward(n, Vol{1, n3, n2, n1}, scratch1, scratch2, norm, scratch2, s)) return -1;
+    const QGeo g = qgeo(n1, n2, n3);
+    k_select<<<(unsigned)g.nblocks(), 256, 0, s>>>(g, x, scratch1, scratch2, code);
+    LC_TRY(hipGetLastError());
+    return 0;
+}
+
+long wavefd_lcodec_gather(const float *src, long n1, long n2, long n3, const long *origins, long npatch, long P,
+                          float scale, float *dst, void *stream) {
+    if (P <= 0 || npatch <= 0 || P > n1 || P > n2 || P > n3) {
+        g_err = "gather: patch larger than the field";
+        return -1;
+    }
+    std::vector<long> org(origins, origins + 3 * npatch);
+    for (long i = 0; i < npatch; i++)
+        if (org[3 * i] < 0 || org[3 * i] + P > n1 || org[3 * i + 1] < 0 || org[3 * i + 1] + P > n2 ||
+            org[3 * i + 2] < 0 || org[3 * i + 2] + P > n3) {
+            g_err = "gather: patch outside the field";
+            return -1;
+        }
+    hipStream_t s = (hipStream_t)stream;
+    long *dorg = nullptr;
+    LC_TRY(hipMalloc((void **)&dorg, sizeof(long) * 3 * npatch));
+    hipError_t e = hipMemcpyAsync(dorg, org.data(), sizeof(long) * 3 * npatch, hipMemcpyHostToDevice, s);
+    const long tot = P * P * P * npatch;
+    if (e == hipSuccess) {
+        k_gather<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(src, n1, n2, n3, dorg, P, npatch, scale, dst);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dorg);
+    if (e != hipSuccess) {
+        g_err = std::string("gather: ") + hipGetErrorString(e);
+        return -1;
+    }
+    return 0;
+}
+
+}  // extern "C"
+#pragma GCC visibility pop
