@@ -297,6 +297,154 @@ def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000), fused=Tr
             else "separate calls"}
 
 
+def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
+    """Learned snapshot codec (wavefd_amd/lcodec.py) on the bench's Iso model construction.
+    Training: DP snapshots (every 3rd step) of an n^3 shot made by the propagator's fused SaveDP
+    steps, decoder trained on the GPU (iters Adam steps of 4 x 32^3 patches).  Evaluation on a
+    second model: snapshot errors of the plain and the learned decode, and one Born shot's image
+    with exact, plain and learned snapshot storage (the image error judged against exact
+    storage).  Rates: encode (with the encoder's choice) and learned decode of a big_n^3 field."""
+    import torch
+    import wavefd_amd as w
+    from wavefd_amd import born, ginsu, lcodec, spacetime as st
+
+    dev = torch.device("cuda:0")
+    hip = _hip()
+    d, dt = 12.5, 1e-3
+    shape = (n, n, n)
+    F32 = np.float32
+
+    def model(sd):
+        m = build_model(torch, "iso", n, d, dt, sd, dev)
+        out = {}
+        for k in ("v", "b"):
+            t = m[k].contiguous()
+            out[k] = ginsu.DeviceBuffer(t.numel() * 4)
+            torch.cuda.synchronize()
+            _copy_d2d(hip, out[k].ptr, t)
+        del m
+        return out
+
+    def snapshots(mdl):
+        p = w.Prop3DAcoIsoDenQ_DEO2_FDTD(nz=n, ny=n, nx=n, dz=d, dy=d, dx=d, dt=dt, nsponge=20, freesurface=True,
+                                         freqQ=10.0, qMin=0.1, qInterior=100.0)
+        w.coherence(p, "device")
+        for k in ("v", "b"):
+            ginsu.copy(w.device_ptr(p, k), mdl[k].ptr, 4 * n ** 3)
+        ginsu.sync()
+        src = st.hickscoeffs(F32(d), F32(d), F32(d), 0.0, 0.0, 0.0, n, n, n, [20.0], [n * d / 2 + 3.0],
+                             [n * d / 2 - 4.0], fs_index=1)
+        w.set_sources(p, [src], st.causal_ricker(15.0, dt * np.arange(nt)).reshape(1, -1).astype(F32))
+        dp = ginsu.DeviceBuffer(4 * n ** 3)
+        out = []
+        strm = p._lib.GetStream(p.p)
+        for it in range(1, nt + 1):
+            p._lib.TimeStepSaveDP(p.p, dp.ptr)
+            p._lib.InjectRegistered(p.p, it)
+            if it % 3 == 0:
+                b = ginsu.DeviceBuffer(4 * n ** 3)
+                ginsu.copy(b.ptr, dp.ptr, 4 * n ** 3, strm)
+                out.append(b)
+        w.synchronize(p)
+        dp.free()
+        w.free(p)
+        return out
+
+    t0 = time.perf_counter()
+    m1 = model(seed)
+    train = snapshots(m1)
+    t_gen = time.perf_counter() - t0
+    codec = lcodec.LearnedCodec(seed=seed)
+    t0 = time.perf_counter()
+    losses = codec.train([b.ptr for b in train], shape, iters=iters, patch=32, batch=4, lr=1e-3, seed=seed)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    for b in train:
+        b.free()
+    # snapshot errors on the second model
+    m2 = model(seed + 1)
+    test = snapshots(m2)
+    cb = ginsu.DeviceBuffer(4 * lcodec.code_words(shape))
+    a, b2, scr = (ginsu.DeviceBuffer(4 * n ** 3) for _ in range(3))
+    ep, el = [], []
+    for sb in test:
+        codec.encode(sb.ptr, shape, cb.ptr, a.ptr, scr.ptr)
+        lcodec.dequantize(cb.ptr, a.ptr)
+        codec.decode(cb.ptr, b2.ptr, scr.ptr)
+        x = sb.download(np.zeros(n ** 3, F32)).astype(np.float64)
+        nx_ = np.linalg.norm(x)
+        ep.append(np.linalg.norm(a.download(np.zeros(n ** 3, F32)) - x) / nx_)
+        el.append(np.linalg.norm(b2.download(np.zeros(n ** 3, F32)) - x) / nx_)
+    for q in test + [cb, a, b2, scr]:
+        q.free()
+    # the Born image with the three snapshot stores
+    dvt = (0.02 * build_model(torch, "iso", n, d, dt, seed + 1, dev)["v"] *
+           _smooth_noise(torch, shape, 15, seed + 300, dev)).contiguous()
+    dvb = ginsu.DeviceBuffer(dvt.numel() * 4)
+    torch.cuda.synchronize()
+    _copy_d2d(hip, dvb.ptr, dvt)
+    del dvt
+    win = ginsu.Window(shape, (0, 0, 0), shape)
+    ry, rx = np.meshgrid(np.linspace(30 * d, (n - 30) * d, 16), np.linspace(30 * d, (n - 30) * d, 16), indexing="ij")
+    shot = born.BornShot((20.0, n * d / 2 + 3.0, n * d / 2 - 4.0), np.full(ry.size, 20.0), ry.ravel(), rx.ravel(),
+                         st.causal_ricker(15.0, dt * np.arange(nt)))
+    imgs, times = {}, {}
+    for mode in ("exact", "plain", "learned"):
+        img = ginsu.DeviceBuffer(4 * n ** 3)
+        img.zero()
+        t0 = time.perf_counter()
+        born.born_shot("iso", win, (d, d, d), dt, shot, {k: q.ptr for k, q in m2.items()}, dvb.ptr, img.ptr,
+                       nsponge=20, snapshot_codec=None if mode == "exact" else codec, learned=mode == "learned")
+        times[mode] = time.perf_counter() - t0
+        imgs[mode] = img.download(np.zeros(n ** 3, F32)).astype(np.float64)
+        img.free()
+    ref_n = np.linalg.norm(imgs["exact"])
+    gerr = {k: float(np.linalg.norm(imgs[k] - imgs["exact"]) / ref_n) for k in ("plain", "learned")}
+    for q in list(m1.values()) + list(m2.values()) + [dvb]:
+        q.free()
+    # rates on a big field
+    bs = (big_n, big_n, big_n)
+    f = _smooth_noise(torch, bs, 4, seed + 500, dev).contiguous()
+    fb = ginsu.DeviceBuffer(f.numel() * 4)
+    torch.cuda.synchronize()
+    _copy_d2d(hip, fb.ptr, f)
+    del f
+    torch.cuda.empty_cache()
+    cb = ginsu.DeviceBuffer(4 * lcodec.code_words(bs))
+    o, s1 = ginsu.DeviceBuffer(4 * big_n ** 3), ginsu.DeviceBuffer(4 * big_n ** 3)
+    codec.encode(fb.ptr, bs, cb.ptr, o.ptr, s1.ptr)  # warm-up (workspace)
+    codec.decode(cb.ptr, o.ptr, s1.ptr)
+    ginsu.sync()
+    t0 = time.perf_counter()
+    codec.encode(fb.ptr, bs, cb.ptr, o.ptr, s1.ptr)
+    ginsu.sync()
+    t_enc = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    codec.decode(cb.ptr, o.ptr, s1.ptr)
+    ginsu.sync()
+    t_dec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lcodec.encode(fb.ptr, bs, cb.ptr)
+    lcodec.dequantize(cb.ptr, o.ptr)
+    ginsu.sync()
+    t_plain = time.perf_counter() - t0
+    for q in (fb, cb, o, s1):
+        q.free()
+    codec.close()
+    gb = 4 * big_n ** 3 / 1e9
+    return {"ratio_512": round(lcodec.ratio((512, 512, 512)), 3), "ratio_window": round(lcodec.ratio(shape), 3),
+            "train": {"snapshots": len(train), "grid": n, "iters": iters, "patch": 32, "batch": 4, "seconds":
+                      round(t_train, 2), "loss_first": float(losses[:20].mean()),
+                      "loss_last": float(losses[-50:].mean()), "generate_seconds": round(t_gen, 2)},
+            "snapshot_rel_err": {"plain": round(float(np.mean(ep)), 4), "learned": round(float(np.mean(el)), 4),
+                                 "snapshots": len(ep)},
+            "born_image_rel_err_vs_exact": {k: round(v, 4) for k, v in gerr.items()},
+            "born_seconds": {k: round(v, 2) for k, v in times.items()},
+            "rates_GBps_512": {"encode_learned": round(gb / t_enc, 1), "decode_learned": round(gb / t_dec, 1),
+                               "encode_plus_dequantize_plain": round(gb / t_plain, 1)},
+            "target": "north star: >= 8x and Born gradient rel L2 <= 1e-3 against exact storage"}
+
+
 def cpu_baseline(physics, n, target_s=12.0):
     """Time the oracle's two-pass step (the reference's structure, nbx=nby=8, nbz=nz) on the
     host cores for about target_s seconds on an n^3 grid; returns Mcells/s."""
@@ -382,6 +530,7 @@ def main():
     ap.add_argument("--born-nt", type=int, default=200)
     ap.add_argument("--born-plain", action="store_true", help="Born with the separate per-step calls")
     ap.add_argument("--born-batch", type=int, default=1, help="Born shots run together (one launch per step)")
+    ap.add_argument("--codec", action="store_true", help="only train and measure the learned snapshot codec")
     ap.add_argument("--coherent", action="store_true",
                     help="also time the reference's host loop through the coherent API (PCIe-inclusive)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -390,6 +539,9 @@ def main():
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.codec:
+        print(json.dumps({"codec": run_codec()}), flush=True)
         return
     if args.born:
         print(json.dumps({"born": run_born(args.born, args.born_shots, args.born_nt, fused=not args.born_plain,
