@@ -712,3 +712,145 @@ void wo_tti_step(long fs, long nx, long ny, long nz, float idx, float idy, float
 }
 
 int wo_max_threads(void) { return omp_get_max_threads(); }
+
+/* ------------------------------------------------------------------------- */
+/* 2D: src/absorb.h:23-75, src/propagatorStaticFunctions.h:10-311,              */
+/*     src/prop2DAcoIsoDenQ_DEO2_FDTD.h:674-1045                                */
+/* Arrays (nx, nz), k = kx*nz + kz.  Outputs are zeroed whole, then the interior */
+/* [4,nx-4) x [4,nz-4) and, under a free surface, the rows kz = 0..3.           */
+/* ------------------------------------------------------------------------- */
+
+/* Returns 0, or 1 when freqQ < FLT_EPSILON (the reference exits there, src/absorb.h:34-39). */
+int wo2_setup_q(long fs, long nx, long nz, long nsponge, float dt, float freqQ, float qMin, float qInterior,
+                float *q) {
+    if (freqQ < FLT_EPSILON) return 1;
+    if ((qMin < FLT_EPSILON) && (qInterior < FLT_EPSILON)) {
+        long nzero = imin(2, nx * nz); /* sizeof(pointer) bytes (src/absorb.h:44) */
+        for (long i = 0; i < nzero; i++) q[i] = 0;
+        return 0;
+    }
+    float *qprof = (float *)malloc(sizeof(float) * (nsponge > 0 ? nsponge : 1));
+    const double lqmin = log((double)qMin);
+    const double lqmax = log((double)qInterior);
+    for (long ks = 0; ks < nsponge; ks++) {
+        const double dk = (double)(ks) / (double)(nsponge - 1);
+        const double lq = lqmin + dk * (lqmax - lqmin);
+        qprof[ks] = expf((float)lq); /* 2D: the ramp in double, exp in float (src/absorb.h:49-55) */
+    }
+    for (long kx = 0; kx < nx; kx++)
+        for (long kz = 0; kz < nz; kz++) {
+            const long ksx = imin(kx, nx - 1 - kx);
+            const long ksz = fs ? (nz - 1 - kz) : imin(kz, nz - 1 - kz);
+            const long ks = imin(ksx, ksz);
+            /* dt * 2.0 is double here (src/absorb.h:67-71) */
+            q[kx * nz + kz] = (float)((double)dt * 2.0 * WPI * (double)freqQ / (double)qInterior);
+            if (ks < nsponge) q[kx * nz + kz] = (float)((double)dt * 2.0 * WPI * (double)freqQ / (double)qprof[ks]);
+        }
+    free(qprof);
+    return 0;
+}
+
+#define FOR2_INTERIOR(nx, nz, ...)                                        \
+    do {                                                                  \
+        _Pragma("omp parallel for schedule(static)")                      \
+        for (long kx = 4; kx < (nx) - 4; kx++)                            \
+            for (long kz = 4; kz < (nz) - 4; kz++) {                      \
+                const long k = kx * (nz) + kz;                            \
+                __VA_ARGS__                                               \
+            }                                                             \
+    } while (0)
+
+#define FOR2_FS_ROWS(nx, nz, ...)                                         \
+    do {                                                                  \
+        _Pragma("omp parallel for schedule(static)")                      \
+        for (long kx = 4; kx < (nx) - 4; kx++)                            \
+            for (long kz = 0; kz < 4; kz++) {                             \
+                const long k = kx * (nz) + kz;                            \
+                const long kc = kx * (nz);                                \
+                __VA_ARGS__                                               \
+            }                                                             \
+    } while (0)
+
+void wo2_dplus(long fs, long nx, long nz, float idx, float idz, const float *inX, const float *inZ, float *outX,
+               float *outZ) {
+    memset(outX, 0, sizeof(float) * nx * nz);
+    memset(outZ, 0, sizeof(float) * nx * nz);
+    FOR2_INTERIOR(nx, nz, {
+        outX[k] = idx * sten_plus(inX, k, nz);
+        outZ[k] = idz * sten_plus(inZ, k, 1);
+    });
+    if (fs) {
+        FOR2_FS_ROWS(nx, nz, {
+            outX[k] = kz == 0 ? 0.0f : idx * sten_plus(inX, k, nz);
+            outZ[k] = idz * sten_plus_fs(inZ + kc, kz);
+        });
+    }
+}
+
+void wo2_dminus(long fs, long nx, long nz, float idx, float idz, const float *inX, const float *inZ, float *outX,
+                float *outZ) {
+    memset(outX, 0, sizeof(float) * nx * nz);
+    memset(outZ, 0, sizeof(float) * nx * nz);
+    FOR2_INTERIOR(nx, nz, {
+        outX[k] = idx * sten_minus(inX, k, nz);
+        outZ[k] = idz * sten_minus(inZ, k, 1);
+    });
+    if (fs) {
+        FOR2_FS_ROWS(nx, nz, {
+            if (kz == 0) {
+                outX[k] = 0;
+                outZ[k] = 0;
+            } else {
+                outX[k] = idx * sten_minus(inX, k, nz);
+                outZ[k] = idz * sten_minus_fs(inZ + kc, kz);
+            }
+        });
+    }
+}
+
+void wo2_iso_sandwich(long fs, long nx, long nz, float idx, float idz, const float *p, const float *b, float *tX,
+                      float *tZ) {
+    memset(tX, 0, sizeof(float) * nx * nz);
+    memset(tZ, 0, sizeof(float) * nx * nz);
+    FOR2_INTERIOR(nx, nz, {
+        const float dPx = idx * sten_plus(p, k, nz);
+        const float dPz = idz * sten_plus(p, k, 1);
+        const float B = b[k];
+        tX[k] = B * dPx;
+        tZ[k] = B * dPz;
+    });
+    if (fs) {
+        FOR2_FS_ROWS(nx, nz, {
+            const float dPx = kz == 0 ? 0.0f : idx * sten_plus(p, k, nz);
+            const float dPz = idz * sten_plus_fs(p + kc, kz);
+            const float B = b[k];
+            tX[k] = B * dPx;
+            tZ[k] = B * dPz;
+        });
+    }
+}
+
+void wo2_iso_update(long fs, long nx, long nz, float idx, float idz, float dt, const float *tX, const float *tZ,
+                    const float *v, const float *b, const float *q, const float *pCur, float *pSpace, float *pOld) {
+    const float dt2 = dt * dt;
+    memset(pSpace, 0, sizeof(float) * nx * nz);
+    FOR2_INTERIOR(nx, nz, {
+        const float dPx = idx * sten_minus(tX, k, nz);
+        const float dPz = idz * sten_minus(tZ, k, 1);
+        const float dt2V2_B = dt2 * v[k] * v[k] / b[k];
+        pOld[k] = dt2V2_B * (dPx + dPz) - q[k] * (pCur[k] - pOld[k]) - pOld[k] + 2 * pCur[k];
+        pSpace[k] = dPx + dPz;
+    });
+    if (fs) {
+        FOR2_FS_ROWS(nx, nz, {
+            float dPx = 0, dPz = 0;
+            if (kz > 0) {
+                dPx = idx * sten_minus(tX, k, nz);
+                dPz = idz * sten_minus_fs(tZ + kc, kz);
+            }
+            const float dt2V2_B = dt2 * v[k] * v[k] / b[k];
+            pOld[k] = dt2V2_B * (dPx + dPz) - q[k] * (pCur[k] - pOld[k]) - pOld[k] + 2 * pCur[k];
+            pSpace[k] = dPx + dPz;
+        });
+    }
+}

@@ -26,7 +26,7 @@ def gpu_available():
 
 def smooth(rng, shape, amp, passes=3):
     x = rng.standard_normal(shape)
-    for ax in range(3):
+    for ax in range(len(shape)):
         for _ in range(passes):
             x = 0.25 * np.roll(x, 1, ax) + 0.5 * x + 0.25 * np.roll(x, -1, ax)
     return (amp * x / np.abs(x).max()).astype(F32)

@@ -1,7 +1,8 @@
 """The C-ABI libraries load and export every symbol include/*.h declares (CPU only; no
 compute calls).  Also pins the reference's symbol set: every extern "C" name of the
-reference's 3D shims and spacetime.cc (listed here from the reference sources,
-src/prop3DAco*DenQ_DEO2_FDTD.cc and src/spacetime.cc:228-330) is exported."""
+reference's 3D shims, its 2D Iso shim and spacetime.cc (listed here from the reference sources,
+src/prop3DAco*DenQ_DEO2_FDTD.cc, src/prop2DAcoIsoDenQ_DEO2_FDTD.cc and src/spacetime.cc:228-330)
+is exported."""
 import ctypes
 import os
 import re
@@ -22,6 +23,7 @@ HEADER_OF = {
     "codec": "wavefd_codec.h",
     "ginsu": "wavefd_ginsu.h",
     "lcodec": "wavefd_lcodec.h",
+    "iso2d": "prop2DAcoIsoDenQ_DEO2_FDTD.h",
 }
 
 REFERENCE_SYMBOLS = {
@@ -40,6 +42,11 @@ REFERENCE_SYMBOLS = {
             "AdjointBornAccumulation_wavefieldsep_V", "AdjointBornAccumulation_VEA", "getNx", "getNy", "getNz",
             "getV", "getEps", "getEta", "getSinTheta", "getCosTheta", "getSinPhi", "getCosPhi", "getB", "getF",
             "getDtOmegaInvQ", "getPSpace", "getMSpace", "getPCur", "getPOld", "getMCur", "getMOld"],
+    "iso2d": ["alloc", "free", "SetupDtOmegaInvQ", "TimeStep", "ScaleSpatialDerivatives",
+              "ForwardBornInjection_V", "ForwardBornInjection_VB", "ForwardBornInjection_B",
+              "AdjointBornAccumulation_V", "AdjointBornAccumulation_VB", "AdjointBornAccumulation_B",
+              "AdjointBornAccumulation_wavefieldsep", "getNx", "getNz", "getV", "getB", "getDtOmegaInvQ",
+              "getPSpace", "getPCur", "getPOld"],
     "spacetime": ["injectdata_float", "injectdata_double", "extractdata_float", "extractdata_double",
                   "interpadjoint_1d_float", "interpadjoint_1d_double", "interpforward_1d_float",
                   "interpforward_1d_double", "interpadjoint_nd_float", "interpadjoint_nd_double",
@@ -52,7 +59,7 @@ def declared(kind):
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text))
     names -= {"extern", "if", "defined"}
-    if kind in _lib.PREFIXES:
+    if kind in _lib.PREFIXES and kind != "iso2d":
         add = open(os.path.join(INC, "wavefd_amd.h")).read()
         add = re.sub(r"/\*.*?\*/", "", add, flags=re.S)
         for nm in re.findall(r"P##_(\w+)\s*\(", add):
@@ -67,7 +74,7 @@ def exported(path):
     return {line.split()[-1] for line in out.splitlines() if " T " in line}
 
 
-ALL = ["iso", "vti", "tti", "spacetime", "codec", "ginsu", "lcodec"]
+ALL = ["iso", "vti", "tti", "spacetime", "codec", "ginsu", "lcodec", "iso2d"]
 
 
 @pytest.mark.parametrize("kind", ALL)
@@ -80,7 +87,7 @@ def test_library_exports_every_declared_symbol(kind):
     assert not missing, missing
 
 
-@pytest.mark.parametrize("kind", ["iso", "vti", "tti", "spacetime"])
+@pytest.mark.parametrize("kind", ["iso", "vti", "tti", "spacetime", "iso2d"])
 def test_reference_symbol_set_is_covered(kind):
     pre = "" if kind == "spacetime" else _lib.PREFIXES[kind] + "_"
     names = {pre + s for s in REFERENCE_SYMBOLS[kind]}

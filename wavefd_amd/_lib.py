@@ -16,6 +16,7 @@ PREFIXES = {
     "iso": "Prop3DAcoIsoDenQ_DEO2_FDTD",
     "vti": "Prop3DAcoVTIDenQ_DEO2_FDTD",
     "tti": "Prop3DAcoTTIDenQ_DEO2_FDTD",
+    "iso2d": "Prop2DAcoIsoDenQ_DEO2_FDTD",
 }
 
 # name -> (restype, argtypes); common to the three propagator libraries
@@ -145,6 +146,36 @@ LCODEC = {
     "wavefd_lcodec_LastError": (ctypes.c_char_p, []),
 }
 
+# the 2D Iso library (src/prop2DAcoIsoDenQ_DEO2_FDTD.cc), names without the prefix
+_ISO2D = {
+    "alloc": (_P, [_L, _L, _L, _L, _L, _F, _F, _F, _L, _L]),
+    "free": (None, [_P]),
+    "SetupDtOmegaInvQ": (None, [_P, _F, _F, _F]),
+    "TimeStep": (None, [_P]),
+    "ScaleSpatialDerivatives": (None, [_P]),
+    "ForwardBornInjection_V": (None, [_P, _FP, _FP]),
+    "ForwardBornInjection_VB": (None, [_P, _FP, _FP, _FP, _FP]),
+    "ForwardBornInjection_B": (None, [_P, _FP, _FP, _FP]),
+    "AdjointBornAccumulation_V": (None, [_P, _FP, _FP]),
+    "AdjointBornAccumulation_VB": (None, [_P, _FP, _FP, _FP, _FP]),
+    "AdjointBornAccumulation_B": (None, [_P, _FP, _FP, _FP]),
+    "AdjointBornAccumulation_wavefieldsep": (None, [_P, _FP, _FP, _L]),
+    "getNx": (_L, [_P]),
+    "getNz": (_L, [_P]),
+    "getV": (_P, [_P]),
+    "getB": (_P, [_P]),
+    "getDtOmegaInvQ": (_P, [_P]),
+    "getPSpace": (_P, [_P]),
+    "getPCur": (_P, [_P]),
+    "getPOld": (_P, [_P]),
+    "SetHostCoherence": (None, [_P, _L]),
+    "SyncToDevice": (None, [_P]),
+    "SyncToHost": (None, [_P]),
+    "GetDevicePtr": (_P, [_P, ctypes.c_char_p]),
+    "LastError": (ctypes.c_char_p, []),
+    "ClearError": (None, []),
+}
+
 _TABLES = {"spacetime": SPACETIME, "codec": CODEC, "ginsu": GINSU, "lcodec": LCODEC}
 _LIBNAMES = {"spacetime": "libspacetime.so", "codec": "libwavefd_codec.so", "ginsu": "libwavefd_ginsu.so",
              "lcodec": "libwavefd_lcodec.so"}
@@ -158,6 +189,8 @@ def signatures(kind):
     if kind in _TABLES:
         return dict(_TABLES[kind])
     pre = PREFIXES[kind]
+    if kind == "iso2d":
+        return {pre + "_" + name: sig for name, sig in _ISO2D.items()}
     out = {}
     for name, sig in list(_COMMON.items()) + list(_PHYS[kind].items()):
         out[pre + "_" + name] = sig

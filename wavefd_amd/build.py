@@ -18,6 +18,7 @@ LIBS = {
     "libprop3DAcoIsoDenQ_DEO2_FDTD.so": ("prop3DAcoIsoDenQ_DEO2_FDTD.hip", ["-lhipfft"]),
     "libprop3DAcoVTIDenQ_DEO2_FDTD.so": ("prop3DAcoVTIDenQ_DEO2_FDTD.hip", ["-lhipfft"]),
     "libprop3DAcoTTIDenQ_DEO2_FDTD.so": ("prop3DAcoTTIDenQ_DEO2_FDTD.hip", ["-lhipfft"]),
+    "libprop2DAcoIsoDenQ_DEO2_FDTD.so": ("prop2DAcoIsoDenQ_DEO2_FDTD.hip", ["-lhipfft"]),
     "libspacetime.so": ("spacetime.hip", []),
     # snapshot codec behind the Compressor's lossy mode (src/compressedio.jl); additive
     "libwavefd_codec.so": ("codec.hip", []),

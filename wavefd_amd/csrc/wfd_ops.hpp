@@ -4,6 +4,7 @@
 #pragma once
 #include <hipfft/hipfft.h>
 
+#include "wfd_2d.hpp"
 #include "wfd_fused.hpp"
 #include "wfd_runtime.hpp"
 
@@ -51,7 +52,7 @@ static inline bool step_two_pass(Prop *p, bool linear) {
 }
 
 static inline bool step_device(Prop *p, bool linear) {
-    bool ok = p->fused ? step_fused(p, linear) : step_two_pass(p, linear);
+    bool ok = p->dim == 2 ? step_2d(p, linear) : (p->fused ? step_fused(p, linear) : step_two_pass(p, linear));
     if (!ok) return false;
     p->swap_p();
     return true;
@@ -568,6 +569,82 @@ static inline bool time_step_batch(Physics phys, void **hs, long n, long epi, fl
         if (ps[i]->stream != s0) WFD_TRY(hipStreamWaitEvent(ps[i]->stream, ps[0]->ev, 0));
     for (long i = 0; i < n; i++) ps[i]->swap_p();
     return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// 2D propagators: Born sources and imaging (src/prop2DAcoIsoDenQ_DEO2_FDTD.h:249-582)
+// ---------------------------------------------------------------------------------------
+
+static inline bool iso2_inj_v(Prop *p, float *dV, float *DP) {
+    if (!p->begin(bit(F_V) | bit(F_PCUR))) return false;
+    Prop::Arg a[2];
+    WFD_ARG(dV, 0, true, false, a[0]);
+    WFD_ARG(DP, 1, true, false, a[1]);
+    k2_iso_inj_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, p->f[F_V].d, a[0].dev, a[1].dev,
+                                                            p->f[F_PCUR].d);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, 2)) return false;
+    return p->end(bit(F_PCUR));
+}
+
+// VB (dV != null) and B (dV == null): l = D-(dB D+ P)
+static inline bool iso2_inj_vb(Prop *p, float *dV, float *dB, float *P, float *DP) {
+    if (!p->begin(bit(F_V) | bit(F_B) | bit(F_PCUR))) return false;
+    Prop::Arg a[4];
+    int na = 0;
+    if (dV) WFD_ARG(dV, 0, true, false, a[na++]);
+    Prop::Arg &adB = a[na];
+    WFD_ARG(dB, 1, true, false, a[na++]);
+    Prop::Arg &aP = a[na];
+    WFD_ARG(P, 2, true, false, a[na++]);
+    Prop::Arg &aDP = a[na];
+    WFD_ARG(DP, 3, true, false, a[na++]);
+    float *t[4];
+    for (int i = 0; i < 4; i++)
+        if (!(t[i] = p->scratch(i))) return false;
+    if (!dplus_2d(p, aP.dev, t[0], t[1])) return false;
+    k2_scale2<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, adB.dev, t[0], t[1]);
+    WFD_LAUNCHED();
+    if (!dminus_2d(p, t[0], t[1], t[2], t[3])) return false;
+    k2_iso_inj_vb<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, p->f[F_V].d, p->f[F_B].d,
+                                                             dV ? a[0].dev : nullptr, adB.dev, aDP.dev, t[2], t[3],
+                                                             p->f[F_PCUR].d);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, na)) return false;
+    return p->end(bit(F_PCUR));
+}
+
+// imaging V (gB null), VB, B (gV null)
+static inline bool iso2_acc(Prop *p, float *gV, float *gB, float *P, float *DP) {
+    if (!p->begin(bit(F_V) | bit(F_B) | bit(F_POLD))) return false;
+    Prop::Arg a[4];
+    int na = 0;
+    Prop::Arg *agV = nullptr, *agB = nullptr, *aP = nullptr;
+    if (gV) {
+        agV = &a[na];
+        WFD_ARG(gV, 0, true, true, a[na++]);
+    }
+    if (gB) {
+        agB = &a[na];
+        WFD_ARG(gB, 1, true, true, a[na++]);
+        aP = &a[na];
+        WFD_ARG(P, 2, true, false, a[na++]);
+    }
+    Prop::Arg &aDP = a[na];
+    WFD_ARG(DP, 3, true, false, a[na++]);
+    float *t[4] = {};
+    if (gB) {
+        for (int i = 0; i < 4; i++)
+            if (!(t[i] = p->scratch(i))) return false;
+        if (!dplus_2d(p, aP->dev, t[0], t[1])) return false;
+        if (!dplus_2d(p, p->f[F_POLD].d, t[2], t[3])) return false;
+    }
+    k2_iso_acc<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->f[F_V].d, p->f[F_B].d, aDP.dev, p->f[F_POLD].d,
+                                                          t[0], t[1], t[2], t[3], agV ? agV->dev : nullptr,
+                                                          agB ? agB->dev : nullptr);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, na)) return false;
+    return p->end(0);
 }
 
 }  // namespace wfd

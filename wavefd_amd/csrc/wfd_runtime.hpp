@@ -186,6 +186,7 @@ struct Prop {
     unsigned *fused_sync = nullptr;  // per-XCD round counters of the persistent fused launch
     hipEvent_t ev = nullptr;         // orders this handle's stream against a batch's (lazy)
     bool born_field = false;         // Iso: mCur/mOld hold the coupled Born step's second field
+    int dim = 3;                     // 2: a 2D propagator (ny = 1, two-pass 2D kernels)
 
     Mask model_mask() const {
         Mask m = bit(F_V) | bit(F_B) | bit(F_Q);

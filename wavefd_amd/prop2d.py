@@ -1,0 +1,167 @@
+"""The 2D propagators' host API, mirroring the reference's Julia wrappers
+(src/prop2DAcoIsoDenQ_DEO2_FDTD.jl) over the drop-in library libprop2DAcoIsoDenQ_DEO2_FDTD.
+
+Arrays are (nz, nx) in Julia's column-major order: numpy views of shape (nz, nx) with Fortran
+order over the library's host mirrors (k = kx nz + kz).  The default host coherence gives the
+reference's synchronous semantics: every call uploads the host arrays it reads and downloads
+the ones it writes.
+"""
+import os
+
+import numpy as np
+
+from . import _lib
+from .prop3d import (ImagingConditionStandard, ImagingConditionWaveFieldSeparationFWI,  # noqa: F401
+                     ImagingConditionWaveFieldSeparationMIX, ImagingConditionWaveFieldSeparationRTM, _ptr)
+
+
+class Prop2DAcoIsoDenQ_DEO2_FDTD:
+    """src/prop2DAcoIsoDenQ_DEO2_FDTD.jl:1-41"""
+    KIND = "iso2d"
+
+    def __init__(self, *, nz=0, nx=0, nsponge=60, nbz=512, nbx=8, dz=5.0, dx=5.0, dt=0.001, nthreads=None,
+                 freesurface=True, freqQ=5.0, qMin=0.1, qInterior=100.0):
+        if nthreads is None:
+            nthreads = os.cpu_count() or 1
+        nz, nx, nsponge, nbz, nbx, nthreads = (int(round(v)) for v in (nz, nx, nsponge, nbz, nbx, nthreads))
+        dz, dx, dt = (np.float32(v) for v in (dz, dx, dt))
+        assert nx > 0
+        assert nz > 0
+        assert nsponge > 0
+        assert nthreads > 0
+        assert nbx > 0
+        assert nbz > 0
+        fs = 1 if freesurface else 0
+        self._lib = _lib.load(self.KIND)
+        self.p = self._lib.alloc(fs, nthreads, nx, nz, nsponge, dx, dz, dt, nbx, nbz)
+        if not self.p:
+            raise RuntimeError("alloc failed: " + _lib.last_error(self.KIND))
+        self._lib.SetupDtOmegaInvQ(self.p, freqQ, qMin, qInterior)
+        self.shape = (nz, nx)
+        self.n = nz * nx
+
+    def __del__(self):
+        try:
+            free(self)
+        except Exception:
+            pass
+
+    def __repr__(self):
+        nz, nx = self.shape
+        return "Prop2DAcoIsoDenQ_DEO2_FDTD -- nx,nz; %d,%d" % (nx, nz)
+
+    def _check(self):
+        msg = _lib.last_error(self.KIND)
+        if msg:
+            self._lib.ClearError()
+            raise RuntimeError(msg)
+
+
+class Prop2DAcoIsoDenQ_DEO2_FDTD_Model_V:
+    pass
+
+
+class Prop2DAcoIsoDenQ_DEO2_FDTD_Model_B:
+    pass
+
+
+class Prop2DAcoIsoDenQ_DEO2_FDTD_Model_VB:
+    pass
+
+
+def free(prop):
+    if getattr(prop, "p", None):
+        prop._lib.free(prop.p)
+        prop.p = None
+
+
+def size(prop):
+    """(nz, nx), src/prop2DAcoIsoDenQ_DEO2_FDTD.jl:45-49"""
+    return (prop._lib.getNz(prop.p), prop._lib.getNx(prop.p))
+
+
+def _wrap(prop, getter):
+    import ctypes
+    addr = getattr(prop._lib, getter)(prop.p)
+    if not addr:
+        prop._check()
+        raise RuntimeError("getter %s returned NULL" % getter)
+    buf = (ctypes.c_float * prop.n).from_address(addr)
+    return np.ndarray(prop.shape, dtype=np.float32, buffer=buf, order="F")
+
+
+def _make_getter(name):
+    def getter(prop):
+        return _wrap(prop, "get" + name)
+
+    getter.__name__ = name
+    getter.__doc__ = "Host view of the library-owned %s array, shape (nz, nx), Fortran order." % name
+    return getter
+
+
+V = _make_getter("V")
+B = _make_getter("B")
+DtOmegaInvQ = _make_getter("DtOmegaInvQ")
+PSpace = _make_getter("PSpace")
+PCur = _make_getter("PCur")
+POld = _make_getter("POld")
+
+
+def propagateforward(prop):
+    prop._lib.TimeStep(prop.p)
+    prop._check()
+
+
+def propagateadjoint(prop):
+    """self-adjoint (src/prop2DAcoIsoDenQ_DEO2_FDTD.jl:61)"""
+    propagateforward(prop)
+
+
+def scale_spatial_derivatives(prop):
+    prop._lib.ScaleSpatialDerivatives(prop.p)
+    prop._check()
+
+
+def forwardBornInjection(prop, modeltype, dmodel, wavefields):
+    """src/prop2DAcoIsoDenQ_DEO2_FDTD.jl:66-85"""
+    L, p, n = prop._lib, prop.p, prop.n
+    name = type(modeltype).__name__
+    if name.endswith("_Model_V"):
+        L.ForwardBornInjection_V(p, _ptr(dmodel["v"], n), _ptr(wavefields["pspace"], n))
+    elif name.endswith("_Model_VB"):
+        L.ForwardBornInjection_VB(p, _ptr(dmodel["v"], n), _ptr(dmodel["b"], n), _ptr(wavefields["pold"], n),
+                                  _ptr(wavefields["pspace"], n))
+    elif name.endswith("_Model_B"):
+        L.ForwardBornInjection_B(p, _ptr(dmodel["b"], n), _ptr(wavefields["pold"], n), _ptr(wavefields["pspace"], n))
+    else:
+        raise TypeError(name)
+    prop._check()
+
+
+def adjointBornAccumulation(prop, modeltype, imagingcondition, dmodel, wavefields):
+    """src/prop2DAcoIsoDenQ_DEO2_FDTD.jl:87-144"""
+    L, p, n = prop._lib, prop.p, prop.n
+    name = type(modeltype).__name__
+    ic = type(imagingcondition).__name__
+    if ic == "ImagingConditionWaveFieldSeparationMIX":
+        keys = {"v": ("all_v", "rtm_v"), "b": ("all_b", "rtm_b")}
+        d_all = {k: dmodel[a] for k, (a, r) in keys.items() if a in dmodel}
+        d_rtm = {k: dmodel[r] for k, (a, r) in keys.items() if r in dmodel}
+        adjointBornAccumulation(prop, modeltype, ImagingConditionStandard(), d_all, wavefields)
+        adjointBornAccumulation(prop, modeltype, ImagingConditionWaveFieldSeparationRTM(), d_rtm, wavefields)
+        return
+    if ic in ("ImagingConditionWaveFieldSeparationFWI", "ImagingConditionWaveFieldSeparationRTM"):
+        if not name.endswith("_Model_V"):
+            raise TypeError("wavefield separation images v only")
+        L.AdjointBornAccumulation_wavefieldsep(p, _ptr(dmodel["v"], n), _ptr(wavefields["pspace"], n),
+                                               1 if ic.endswith("FWI") else 0)
+    elif name.endswith("_Model_V"):
+        L.AdjointBornAccumulation_V(p, _ptr(dmodel["v"], n), _ptr(wavefields["pspace"], n))
+    elif name.endswith("_Model_VB"):
+        L.AdjointBornAccumulation_VB(p, _ptr(dmodel["v"], n), _ptr(dmodel["b"], n), _ptr(wavefields["pold"], n),
+                                     _ptr(wavefields["pspace"], n))
+    elif name.endswith("_Model_B"):
+        L.AdjointBornAccumulation_B(p, _ptr(dmodel["b"], n), _ptr(wavefields["pold"], n), _ptr(wavefields["pspace"], n))
+    else:
+        raise TypeError(name)
+    prop._check()
