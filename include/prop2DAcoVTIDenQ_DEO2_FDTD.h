@@ -1,0 +1,78 @@
+/* libprop2DAcoVTIDenQ_DEO2_FDTD -- MI355X drop-in for the reference's 2D VTI library
+ * (src/prop2DAcoVTIDenQ_DEO2_FDTD.cc).  Same names, argument order and C types; arrays
+ * (nz, nx) float32, Julia column-major, k = kx nz + kz.  Calls are synchronous and getters
+ * return host arrays kept coherent with the device (the additive coherence switch at the end
+ * makes calls stream-ordered instead). */
+#ifndef PROP2DACOVTIDENQ_DEO2_FDTD_AMD_H
+#define PROP2DACOVTIDENQ_DEO2_FDTD_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/prop2DAcoVTIDenQ_DEO2_FDTD.cc:6-23 */
+void *Prop2DAcoVTIDenQ_DEO2_FDTD_alloc(long fs, long nthread, long nx, long nz, long nsponge, float dx, float dz,
+                                       float dt, long nbx, long nbz);
+/* :25-28 */
+void Prop2DAcoVTIDenQ_DEO2_FDTD_free(void *p);
+/* :30-39 (src/absorb.h:23-75; exits the process on freqQ < FLT_EPSILON, like the reference) */
+void Prop2DAcoVTIDenQ_DEO2_FDTD_SetupDtOmegaInvQ(void *p, float freqQ, float qMin, float qInterior);
+/* :41-44 (no pSpace/mSpace) and :46-49 */
+void Prop2DAcoVTIDenQ_DEO2_FDTD_TimeStepLinear(void *p);
+void Prop2DAcoVTIDenQ_DEO2_FDTD_TimeStep(void *p);
+/* :51-54 */
+void Prop2DAcoVTIDenQ_DEO2_FDTD_ScaleSpatialDerivatives(void *p);
+/* :56-60, :62-67 */
+void Prop2DAcoVTIDenQ_DEO2_FDTD_ForwardBornInjection_V(void *p, float *dVel, float *wavefieldDP, float *wavefieldDM);
+void Prop2DAcoVTIDenQ_DEO2_FDTD_ForwardBornInjection_VEA(void *p, float *dVel, float *dEps, float *dEta,
+                                                         float *wavefieldP, float *wavefieldM, float *wavefieldDP,
+                                                         float *wavefieldDM);
+/* :69-73, :75-79 (hipFFT), :81-86 */
+void Prop2DAcoVTIDenQ_DEO2_FDTD_AdjointBornAccumulation_V(void *p, float *dVel, float *wavefieldDP,
+                                                          float *wavefieldDM);
+void Prop2DAcoVTIDenQ_DEO2_FDTD_AdjointBornAccumulation_wavefieldsep_V(void *p, float *dVel, float *wavefieldDP,
+                                                                       float *wavefieldDM, const long isFWI);
+void Prop2DAcoVTIDenQ_DEO2_FDTD_AdjointBornAccumulation_VEA(void *p, float *dVel, float *dEps, float *dEta,
+                                                            float *wavefieldP, float *wavefieldM, float *wavefieldDP,
+                                                            float *wavefieldDM);
+/* :88-96 */
+long Prop2DAcoVTIDenQ_DEO2_FDTD_getNx(void *p);
+long Prop2DAcoVTIDenQ_DEO2_FDTD_getNz(void *p);
+/* :98-156 */
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_getV(void *p);
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_getEps(void *p);
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_getEta(void *p);
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_getB(void *p);
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_getF(void *p);
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_getDtOmegaInvQ(void *p);
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_getPSpace(void *p);
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_getMSpace(void *p);
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_getPCur(void *p);
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_getPOld(void *p);
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_getMCur(void *p);
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_getMOld(void *p);
+/* :158-176: the stateless D+ on host arrays with the caller's coefficients and spacings
+ * (nthread and the block sizes are accepted and unused) */
+void Prop2DAcoVTIDenQ_DEO2_FDTD_applyFirstDerivatives2D_PlusHalf(const long freeSurface, const long nx, const long nz,
+                                                                 const long nthread, const float c8_1,
+                                                                 const float c8_2, const float c8_3, const float c8_4,
+                                                                 const float invDx, const float invDz,
+                                                                 const float *const inX, const float *const inZ,
+                                                                 float *outX, float *outZ, const long BX_2D,
+                                                                 const long BZ_2D);
+
+/* additive, no counterpart in the reference: host coherence (1 coherent, 0 device), explicit
+ * copies, device addresses by name (v b dtOmegaInvQ eps eta f pCur pOld mCur mOld pSpace
+ * mSpace), errors */
+void Prop2DAcoVTIDenQ_DEO2_FDTD_SetHostCoherence(void *p, long mode);
+void Prop2DAcoVTIDenQ_DEO2_FDTD_SyncToDevice(void *p);
+void Prop2DAcoVTIDenQ_DEO2_FDTD_SyncToHost(void *p);
+float *Prop2DAcoVTIDenQ_DEO2_FDTD_GetDevicePtr(void *p, const char *name);
+const char *Prop2DAcoVTIDenQ_DEO2_FDTD_LastError(void);
+void Prop2DAcoVTIDenQ_DEO2_FDTD_ClearError(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

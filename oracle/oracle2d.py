@@ -7,11 +7,19 @@ profile of src/absorb.h:23-75) and composes the Born sources and imaging conditi
 Arrays are C-ordered (nx, nz) float32, i.e. k = kx*nz + kz (Julia's (nz, nx) column-major view).
 The product package never imports it.
 
+The VTI and TTI classes restate src/prop2DAcoVTIDenQ_DEO2_FDTD.h:16-1478 and
+src/prop2DAcoTTIDenQ_DEO2_FDTD.h:14-1717 the same way (``wo2_vti_*``, ``wo2_tti_*``).
+
 Pins (tests/test_oracle2d.py): D+/D- exact on polynomials of degree <= 7 along x and z and
 discretely adjoint (<D+u, w> = -<u, D-w>) with and without the free-surface mirrors; the Iso
 step equals the 3D oracle's on an (nx, 9, nz) grid whose y-derivatives vanish (a field
-constant in y) in the rows the 3D interior covers; energy conservation of the Q = 0 leapfrog;
-the Q profile against its closed form.
+constant in y) in the rows the 3D interior covers; the Q profile against its closed form;
+VTI: eps = eta = 0 gives the Iso step on P bit for bit, and the step equals the 3D VTI
+oracle's on y-constant fields (bit for bit at eta = 0, where the 2D and 3D texts' different
+grouping of the coupling term cannot show, and to 1e-6 otherwise); TTI: theta = 0 gives the
+VTI step bit for bit, theta = 90 degrees the VTI step on the transposed grid bit for bit, and
+the rotated D+ and D- are negative adjoints; every Born source is the finite-difference
+derivative of the step and every imaging condition its adjoint.
 """
 import ctypes
 
@@ -34,11 +42,19 @@ def lib():
         l.wo2_dminus.argtypes = d
         l.wo2_iso_sandwich.argtypes = [_L, _L, _L, _F, _F, _fp, _fp, _fp, _fp]
         l.wo2_iso_update.argtypes = [_L, _L, _L, _F, _F, _F] + [_fp] * 8
+        l.wo2_vti_sandwich.argtypes = [_L, _L, _L, _F, _F] + [_fp] * 10
+        l.wo2_vti_update.argtypes = [_L, _L, _L, _F, _F, _F] + [_fp] * 13
+        l.wo2_tti_dplus.argtypes = [_L, _L, _L, _F, _F] + [_fp] * 6
+        l.wo2_tti_dminus.argtypes = [_L, _L, _L, _F, _F] + [_fp] * 6
+        l.wo2_tti_sandwich.argtypes = [_L, _L, _L, _F, _F] + [_fp] * 12
+        l.wo2_tti_update.argtypes = [_L, _L, _L, _F, _F, _F] + [_fp] * 15
         l._wo2_declared = True
     return l
 
 
 def _p(a):
+    if a is None:
+        return None
     assert a.dtype == np.float32 and a.flags.c_contiguous
     return a.ctypes.data_as(_fp)
 
@@ -147,3 +163,140 @@ class Oracle2DIso:
 
     def adjoint_born_accumulation_B(self, gB, P, DP):
         self._acc(None, gB, P, DP)
+
+
+class Oracle2DVTI(Oracle2DIso):
+    """Prop2DAcoVTIDenQ_DEO2_FDTD (src/prop2DAcoVTIDenQ_DEO2_FDTD.h:16-1478)"""
+
+    def __init__(self, nx, nz, dx, dz, dt, **kw):
+        super().__init__(nx, nz, dx, dz, dt, **kw)
+        for name in ("eps", "eta", "f", "mCur", "mOld", "mSpace"):
+            setattr(self, name, np.zeros((nx, nz), F32))
+
+    def _z(self):
+        return np.empty_like(self.v)
+
+    def _sandwich(self):
+        g = self.g
+        t = [self._z() for _ in range(4)]
+        lib().wo2_vti_sandwich(*g.head(), _p(self.pCur), _p(self.mCur), _p(self.eps), _p(self.eta), _p(self.f),
+                               _p(self.b), *[_p(a) for a in t])
+        return t
+
+    def _update(self, t, linear):
+        g = self.g
+        lib().wo2_vti_update(*g.head(), self.dt, *[_p(a) for a in t], _p(self.v), _p(self.b), _p(self.q),
+                             _p(self.pCur), _p(self.mCur), None if linear else _p(self.pSpace),
+                             None if linear else _p(self.mSpace), _p(self.pOld), _p(self.mOld))
+
+    def time_step(self, linear=False):
+        """:263-314 (linear: timeStepLinear)"""
+        self._update(self._sandwich(), linear)
+        self.pCur, self.pOld = self.pOld, self.pCur
+        self.mCur, self.mOld = self.mOld, self.mCur
+
+    def time_step_linear(self):
+        self.time_step(linear=True)
+
+    def scale_spatial_derivatives(self):
+        """:320-338"""
+        v2OverB = self.v * self.v / self.b
+        self.pSpace *= v2OverB
+        self.mSpace *= v2OverB
+
+    def forward_born_injection_V(self, dV, DP, DM):
+        """:351-375"""
+        with np.errstate(all="ignore"):
+            factor = 2 * self.dt * self.dt * dV / self.v
+            self.pCur += factor * DP
+            self.mCur += factor * DM
+
+    def _dplus(self, a):
+        return dplus(self.g, a, a)
+
+    def _dminus(self, x, z):
+        return dminus(self.g, x, z)
+
+    def forward_born_injection_VEA(self, dV, dE, dA, P, M, DP, DM):
+        """:383-462"""
+        tPx1, tPz1 = self._dplus(P)
+        tMx1, tMz1 = self._dplus(M)
+        with np.errstate(all="ignore"):
+            A, B, F = self.eta, self.b, self.f
+            tPx2 = (2 * B * dE) * tPx1
+            tPz2 = (-2 * B * F * A * dA) * tPz1 + (dA * B * F * (1 - 2 * A * A) / np.sqrt(1 - A * A)) * tMz1
+            tMx2 = np.zeros_like(tPx1)
+            tMz2 = (2 * B * F * A * dA) * tMz1 + (dA * B * F * (1 - 2 * A * A) / np.sqrt(1 - A * A)) * tPz1
+        tPx1, tPz1 = self._dminus(tPx2, tPz2)
+        tMx1, tMz1 = self._dminus(tMx2, tMz2)
+        with np.errstate(all="ignore"):
+            V = self.v
+            dt2v2OverB = self.dt * self.dt * V * V / B
+            factor = 2 * B * dV / (V * V * V)
+            self.pCur += dt2v2OverB * (factor * DP + tPx1 + tPz1)
+            self.mCur += dt2v2OverB * (factor * DM + tMx1 + tMz1)
+
+    def adjoint_born_accumulation_V(self, gV, DP, DM):
+        """:473-492"""
+        with np.errstate(all="ignore"):
+            factor = 2 * self.b / (self.v * self.v * self.v)
+            gV += factor * (DP * self.pOld + DM * self.mOld)
+
+    def adjoint_born_accumulation_VEA(self, gV, gE, gA, P, M, DP, DM):
+        """:629-686"""
+        tPx1, tPz1 = self._dplus(P)
+        tMx1, tMz1 = self._dplus(M)
+        tPx2, tPz2 = self._dplus(self.pOld)
+        tMx2, tMz2 = self._dplus(self.mOld)
+        with np.errstate(all="ignore"):
+            V, A, B, F = self.v, self.eta, self.b, self.f
+            factor = 2 * B / (V * V * V)
+            gV += factor * DP * self.pOld + factor * DM * self.mOld
+            gE += -2 * B * tPx1 * tPx2
+            c = B * F * (1 - 2 * A * A) / np.sqrt(1 - A * A)
+            partP = 2 * B * F * A * tPz1 - c * tMz1
+            partM = 2 * B * F * A * tMz1 + c * tPz1
+            gA += partP * tPz2 - partM * tMz2
+
+
+class Oracle2DTTI(Oracle2DVTI):
+    """Prop2DAcoTTIDenQ_DEO2_FDTD (src/prop2DAcoTTIDenQ_DEO2_FDTD.h:14-1717)"""
+
+    def __init__(self, nx, nz, dx, dz, dt, **kw):
+        super().__init__(nx, nz, dx, dz, dt, **kw)
+        self.sinTheta = np.zeros((nx, nz), F32)
+        self.cosTheta = np.zeros((nx, nz), F32)
+
+    def _sandwich(self):
+        g = self.g
+        t = [self._z() for _ in range(4)]
+        lib().wo2_tti_sandwich(*g.head(), _p(self.pCur), _p(self.mCur), _p(self.sinTheta), _p(self.cosTheta),
+                               _p(self.eps), _p(self.eta), _p(self.f), _p(self.b), *[_p(a) for a in t])
+        return t
+
+    def _update(self, t, linear):
+        assert not linear, "the 2D TTI propagator has no linear step"
+        g = self.g
+        lib().wo2_tti_update(*g.head(), self.dt, *[_p(a) for a in t], _p(self.sinTheta), _p(self.cosTheta),
+                             _p(self.v), _p(self.b), _p(self.q), _p(self.pCur), _p(self.mCur), _p(self.pSpace),
+                             _p(self.mSpace), _p(self.pOld), _p(self.mOld))
+
+    def _dplus(self, a):
+        return tti_dplus(self.g, a, a, self.sinTheta, self.cosTheta)
+
+    def _dminus(self, x, z):
+        return tti_dminus(self.g, x, z, self.sinTheta, self.cosTheta)
+
+
+def tti_dplus(g, g1, g3, sT, cT):
+    """rotated D+ (src/prop2DAcoTTIDenQ_DEO2_FDTD.h:1366-1528)"""
+    o1, o3 = np.empty_like(g1), np.empty_like(g3)
+    lib().wo2_tti_dplus(*g.head(), _p(g1), _p(g3), _p(sT), _p(cT), _p(o1), _p(o3))
+    return o1, o3
+
+
+def tti_dminus(g, g1, g3, sT, cT):
+    """rotated D- (src/prop2DAcoTTIDenQ_DEO2_FDTD.h:1531-1716)"""
+    o1, o3 = np.empty_like(g1), np.empty_like(g3)
+    lib().wo2_tti_dminus(*g.head(), _p(g1), _p(g3), _p(sT), _p(cT), _p(o1), _p(o3))
+    return o1, o3

@@ -854,3 +854,185 @@ void wo2_iso_update(long fs, long nx, long nz, float idx, float idz, float dt, c
         });
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* 2D VTI: src/prop2DAcoVTIDenQ_DEO2_FDTD.h:686-1477                           */
+/* 2D TTI: src/prop2DAcoTTIDenQ_DEO2_FDTD.h:667-1717                           */
+/* ------------------------------------------------------------------------- */
+
+/* The 2D material sandwich (VTI :785-793; TTI :775-783 on the rotated derivatives).  Unlike
+ * the 3D text, the coupling coefficient multiplies left to right: B * F * A * sqrt(1 - A*A). */
+static inline void aniso2_material(float dPx, float dPz, float dMx, float dMz, float eps, float eta, float f,
+                                   float b, float *tPx, float *tPz, float *tMx, float *tMz) {
+    const float E = 1 + 2 * eps;
+    const float A = eta;
+    const float F = f;
+    const float B = b;
+    *tPx = B * E * dPx;
+    *tPz = B * (1 - F * A * A) * dPz + B * F * A * sqrtf(1 - A * A) * dMz;
+    *tMx = B * (1 - F) * dMx;
+    *tMz = B * F * A * sqrtf(1 - A * A) * dPz + B * (1 - F + F * A * A) * dMz;
+}
+
+void wo2_vti_sandwich(long fs, long nx, long nz, float idx, float idz, const float *p, const float *m,
+                      const float *eps, const float *eta, const float *f, const float *b, float *tPx, float *tPz,
+                      float *tMx, float *tMz) {
+    float *outs[4] = {tPx, tPz, tMx, tMz};
+    for (int a = 0; a < 4; a++) memset(outs[a], 0, sizeof(float) * nx * nz);
+    FOR2_INTERIOR(nx, nz, {
+        const float dPx = idx * sten_plus(p, k, nz);
+        const float dPz = idz * sten_plus(p, k, 1);
+        const float dMx = idx * sten_plus(m, k, nz);
+        const float dMz = idz * sten_plus(m, k, 1);
+        aniso2_material(dPx, dPz, dMx, dMz, eps[k], eta[k], f[k], b[k], &tPx[k], &tPz[k], &tMx[k], &tMz[k]);
+    });
+    if (fs) {
+        FOR2_FS_ROWS(nx, nz, {
+            const float dPx = kz == 0 ? 0.0f : idx * sten_plus(p, k, nz);
+            const float dPz = idz * sten_plus_fs(p + kc, kz);
+            const float dMx = kz == 0 ? 0.0f : idx * sten_plus(m, k, nz);
+            const float dMz = idz * sten_plus_fs(m + kc, kz);
+            aniso2_material(dPx, dPz, dMx, dMz, eps[k], eta[k], f[k], b[k], &tPx[k], &tPz[k], &tMx[k], &tMz[k]);
+        });
+    }
+}
+
+/* P/M D- and leapfrog update (VTI :1032-1259 nonlinear, :1261-1474 linear).  pSpace/mSpace
+ * NULL selects the linear variant, which neither clears nor writes them. */
+void wo2_vti_update(long fs, long nx, long nz, float idx, float idz, float dt, const float *tPx, const float *tPz,
+                    const float *tMx, const float *tMz, const float *v, const float *b, const float *q,
+                    const float *pCur, const float *mCur, float *pSpace, float *mSpace, float *pOld, float *mOld) {
+    const float dt2 = dt * dt;
+    if (pSpace) {
+        memset(pSpace, 0, sizeof(float) * nx * nz);
+        memset(mSpace, 0, sizeof(float) * nx * nz);
+    }
+    FOR2_INTERIOR(nx, nz, {
+        const float dPx = idx * sten_minus(tPx, k, nz);
+        const float dPz = idz * sten_minus(tPz, k, 1);
+        const float dMx = idx * sten_minus(tMx, k, nz);
+        const float dMz = idz * sten_minus(tMz, k, 1);
+        const float dt2V2_B = dt2 * v[k] * v[k] / b[k];
+        const float LP = dPx + dPz;
+        const float LM = dMx + dMz;
+        if (pSpace) { pSpace[k] = LP; mSpace[k] = LM; }
+        pOld[k] = dt2V2_B * LP - q[k] * (pCur[k] - pOld[k]) - pOld[k] + 2 * pCur[k];
+        mOld[k] = dt2V2_B * LM - q[k] * (mCur[k] - mOld[k]) - mOld[k] + 2 * mCur[k];
+    });
+    if (fs) {
+        FOR2_FS_ROWS(nx, nz, {
+            float dPx = 0, dPz = 0, dMx = 0, dMz = 0;
+            if (kz > 0) {
+                dPx = idx * sten_minus(tPx, k, nz);
+                dPz = idz * sten_minus_fs(tPz + kc, kz);
+                dMx = idx * sten_minus(tMx, k, nz);
+                dMz = idz * sten_minus_fs(tMz + kc, kz);
+            }
+            const float dt2V2_B = dt2 * v[k] * v[k] / b[k];
+            const float LP = dPx + dPz;
+            const float LM = dMx + dMz;
+            if (pSpace) { pSpace[k] = LP; mSpace[k] = LM; }
+            pOld[k] = dt2V2_B * LP - q[k] * (pCur[k] - pOld[k]) - pOld[k] + 2 * pCur[k];
+            mOld[k] = dt2V2_B * LM - q[k] * (mCur[k] - mOld[k]) - mOld[k] + 2 * mCur[k];
+        });
+    }
+}
+
+/* TTI D+ with the rotation into the symmetry axis' frame (:1366-1528):
+ * g1 = cos dx - sin dz, g3 = sin dx + cos dz; at the free surface g1 = -sin dz, g3 = cos dz. */
+void wo2_tti_dplus(long fs, long nx, long nz, float idx, float idz, const float *g1, const float *g3,
+                   const float *sT, const float *cT, float *o1, float *o3) {
+    memset(o1, 0, sizeof(float) * nx * nz);
+    memset(o3, 0, sizeof(float) * nx * nz);
+    FOR2_INTERIOR(nx, nz, {
+        const float dx = idx * sten_plus(g1, k, nz);
+        const float dz = idz * sten_plus(g3, k, 1);
+        o1[k] = cT[k] * dx - sT[k] * dz;
+        o3[k] = sT[k] * dx + cT[k] * dz;
+    });
+    if (fs) {
+        FOR2_FS_ROWS(nx, nz, {
+            const float dz = idz * sten_plus_fs(g3 + kc, kz);
+            if (kz == 0) {
+                o1[k] = -sT[k] * dz;
+                o3[k] = +cT[k] * dz;
+            } else {
+                const float dx = idx * sten_plus(g1, k, nz);
+                o1[k] = cT[k] * dx - sT[k] * dz;
+                o3[k] = sT[k] * dx + cT[k] * dz;
+            }
+        });
+    }
+}
+
+/* TTI D- with the transposed rotation applied inside the differences (:1531-1716):
+ * o1 = D-x(cos g1) - D-z(sin g1), o3 = D-x(sin g3) + D-z(cos g3); 0 at the free surface. */
+static inline void tti2_minus(const float *g1, const float *g3, const float *sT, const float *cT, long k, long kc,
+                              long kz, long nz, float idx, float idz, int fsrow, float *o1, float *o3) {
+    const float A1 = sten_minus_w(g1, cT, NULL, k, nz);
+    const float B1 = fsrow ? sten_minus_w_fs(g1, sT, kc, kz) : sten_minus_w(g1, sT, NULL, k, 1);
+    const float A3 = sten_minus_w(g3, sT, NULL, k, nz);
+    const float B3 = fsrow ? sten_minus_w_fs(g3, cT, kc, kz) : sten_minus_w(g3, cT, NULL, k, 1);
+    *o1 = idx * A1 - idz * B1;
+    *o3 = idx * A3 + idz * B3;
+}
+
+void wo2_tti_dminus(long fs, long nx, long nz, float idx, float idz, const float *g1, const float *g3,
+                    const float *sT, const float *cT, float *o1, float *o3) {
+    memset(o1, 0, sizeof(float) * nx * nz);
+    memset(o3, 0, sizeof(float) * nx * nz);
+    FOR2_INTERIOR(nx, nz, { tti2_minus(g1, g3, sT, cT, k, kx * nz, kz, nz, idx, idz, 0, &o1[k], &o3[k]); });
+    if (fs) {
+        FOR2_FS_ROWS(nx, nz, {
+            if (kz == 0) {
+                o1[k] = 0;
+                o3[k] = 0;
+            } else {
+                tti2_minus(g1, g3, sT, cT, k, kc, kz, nz, idx, idz, 1, &o1[k], &o3[k]);
+            }
+        });
+    }
+}
+
+/* TTI sandwich (:667-994): rotated D+ of P and M, then the material */
+void wo2_tti_sandwich(long fs, long nx, long nz, float idx, float idz, const float *p, const float *m,
+                      const float *sT, const float *cT, const float *eps, const float *eta, const float *f,
+                      const float *b, float *tP1, float *tP3, float *tM1, float *tM3) {
+    wo2_tti_dplus(fs, nx, nz, idx, idz, p, p, sT, cT, tP1, tP3);
+    wo2_tti_dplus(fs, nx, nz, idx, idz, m, m, sT, cT, tM1, tM3);
+    const long zlo = fs ? 0 : 4;
+#pragma omp parallel for schedule(static)
+    for (long kx = 4; kx < nx - 4; kx++)
+        for (long kz = zlo; kz < nz - 4; kz++) {
+            const long k = kx * nz + kz;
+            aniso2_material(tP1[k], tP3[k], tM1[k], tM3[k], eps[k], eta[k], f[k], b[k], &tP1[k], &tP3[k], &tM1[k],
+                            &tM3[k]);
+        }
+}
+
+/* TTI D- and leapfrog update (:997-1363), nonlinear only (the 2D TTI has no linear step) */
+void wo2_tti_update(long fs, long nx, long nz, float idx, float idz, float dt, const float *tP1, const float *tP3,
+                    const float *tM1, const float *tM3, const float *sT, const float *cT, const float *v,
+                    const float *b, const float *q, const float *pCur, const float *mCur, float *pSpace,
+                    float *mSpace, float *pOld, float *mOld) {
+    const float dt2 = dt * dt;
+    memset(pSpace, 0, sizeof(float) * nx * nz);
+    memset(mSpace, 0, sizeof(float) * nx * nz);
+    const long zlo = fs ? 0 : 4;
+#pragma omp parallel for schedule(static)
+    for (long kx = 4; kx < nx - 4; kx++)
+        for (long kz = zlo; kz < nz - 4; kz++) {
+            const long k = kx * nz + kz;
+            float dP1 = 0, dP3 = 0, dM1 = 0, dM3 = 0;
+            if (!(fs && kz == 0)) {
+                const int fsrow = fs && kz < 4;
+                tti2_minus(tP1, tP3, sT, cT, k, kx * nz, kz, nz, idx, idz, fsrow, &dP1, &dP3);
+                tti2_minus(tM1, tM3, sT, cT, k, kx * nz, kz, nz, idx, idz, fsrow, &dM1, &dM3);
+            }
+            const float dt2V2_B = dt2 * v[k] * v[k] / b[k];
+            pSpace[k] = dP1 + dP3;
+            mSpace[k] = dM1 + dM3;
+            pOld[k] = dt2V2_B * pSpace[k] - q[k] * (pCur[k] - pOld[k]) - pOld[k] + 2 * pCur[k];
+            mOld[k] = dt2V2_B * mSpace[k] - q[k] * (mCur[k] - mOld[k]) - mOld[k] + 2 * mCur[k];
+        }
+}

@@ -1,7 +1,7 @@
 """The C-ABI libraries load and export every symbol include/*.h declares (CPU only; no
 compute calls).  Also pins the reference's symbol set: every extern "C" name of the
-reference's 3D shims, its 2D Iso shim and spacetime.cc (listed here from the reference sources,
-src/prop3DAco*DenQ_DEO2_FDTD.cc, src/prop2DAcoIsoDenQ_DEO2_FDTD.cc and src/spacetime.cc:228-330)
+reference's 3D and 2D shims and spacetime.cc (listed here from the reference sources,
+src/prop3DAco*DenQ_DEO2_FDTD.cc, src/prop2DAco*DenQ_DEO2_FDTD.cc and src/spacetime.cc:228-330)
 is exported."""
 import ctypes
 import os
@@ -24,6 +24,8 @@ HEADER_OF = {
     "ginsu": "wavefd_ginsu.h",
     "lcodec": "wavefd_lcodec.h",
     "iso2d": "prop2DAcoIsoDenQ_DEO2_FDTD.h",
+    "vti2d": "prop2DAcoVTIDenQ_DEO2_FDTD.h",
+    "tti2d": "prop2DAcoTTIDenQ_DEO2_FDTD.h",
 }
 
 REFERENCE_SYMBOLS = {
@@ -47,6 +49,16 @@ REFERENCE_SYMBOLS = {
               "AdjointBornAccumulation_V", "AdjointBornAccumulation_VB", "AdjointBornAccumulation_B",
               "AdjointBornAccumulation_wavefieldsep", "getNx", "getNz", "getV", "getB", "getDtOmegaInvQ",
               "getPSpace", "getPCur", "getPOld"],
+    "vti2d": ["alloc", "free", "SetupDtOmegaInvQ", "TimeStepLinear", "TimeStep", "ScaleSpatialDerivatives",
+              "ForwardBornInjection_V", "ForwardBornInjection_VEA", "AdjointBornAccumulation_V",
+              "AdjointBornAccumulation_wavefieldsep_V", "AdjointBornAccumulation_VEA", "getNx", "getNz", "getV",
+              "getEps", "getEta", "getB", "getF", "getDtOmegaInvQ", "getPSpace", "getMSpace", "getPCur",
+              "getPOld", "getMCur", "getMOld", "applyFirstDerivatives2D_PlusHalf"],
+    "tti2d": ["alloc", "free", "SetupDtOmegaInvQ", "TimeStep", "ScaleSpatialDerivatives",
+              "ForwardBornInjection_V", "ForwardBornInjection_VEA", "AdjointBornAccumulation_V",
+              "AdjointBornAccumulation_wavefieldsep_V", "AdjointBornAccumulation_VEA", "getNx", "getNz", "getV",
+              "getEps", "getEta", "getSinTheta", "getCosTheta", "getB", "getF", "getDtOmegaInvQ", "getPSpace",
+              "getMSpace", "getPCur", "getPOld", "getMCur", "getMOld"],
     "spacetime": ["injectdata_float", "injectdata_double", "extractdata_float", "extractdata_double",
                   "interpadjoint_1d_float", "interpadjoint_1d_double", "interpforward_1d_float",
                   "interpforward_1d_double", "interpadjoint_nd_float", "interpadjoint_nd_double",
@@ -59,7 +71,7 @@ def declared(kind):
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text))
     names -= {"extern", "if", "defined"}
-    if kind in _lib.PREFIXES and kind != "iso2d":
+    if kind in _lib.PREFIXES and not kind.endswith("2d"):
         add = open(os.path.join(INC, "wavefd_amd.h")).read()
         add = re.sub(r"/\*.*?\*/", "", add, flags=re.S)
         for nm in re.findall(r"P##_(\w+)\s*\(", add):
@@ -74,7 +86,7 @@ def exported(path):
     return {line.split()[-1] for line in out.splitlines() if " T " in line}
 
 
-ALL = ["iso", "vti", "tti", "spacetime", "codec", "ginsu", "lcodec", "iso2d"]
+ALL = ["iso", "vti", "tti", "spacetime", "codec", "ginsu", "lcodec", "iso2d", "vti2d", "tti2d"]
 
 
 @pytest.mark.parametrize("kind", ALL)
@@ -87,7 +99,7 @@ def test_library_exports_every_declared_symbol(kind):
     assert not missing, missing
 
 
-@pytest.mark.parametrize("kind", ["iso", "vti", "tti", "spacetime", "iso2d"])
+@pytest.mark.parametrize("kind", ["iso", "vti", "tti", "spacetime", "iso2d", "vti2d", "tti2d"])
 def test_reference_symbol_set_is_covered(kind):
     pre = "" if kind == "spacetime" else _lib.PREFIXES[kind] + "_"
     names = {pre + s for s in REFERENCE_SYMBOLS[kind]}

@@ -141,3 +141,153 @@ def test_wavefield_separation(fwi):
         want[kx] = 2 * b[kx] / v[kx] ** 3 * np.real(a2[:nz] * r2[:nz])
     err = np.linalg.norm(g - want) / np.linalg.norm(want)
     assert err <= 2e-5, err
+
+
+# ---- 2D VTI / TTI ---------------------------------------------------------------------------
+
+_GET = {"v": "V", "b": "B", "eps": "Eps", "eta": "Eta", "f": "F", "sinTheta": "SinTheta", "cosTheta": "CosTheta",
+        "pCur": "PCur", "pOld": "POld", "mCur": "MCur", "mOld": "MOld", "pSpace": "PSpace", "mSpace": "MSpace",
+        "q": "DtOmegaInvQ"}
+
+
+def _apair(kind, shape, fs, seed):
+    import wavefd_amd.prop2d as w2
+    from tests.helpers import make_model
+    nx, nz = shape
+    cls = {"vti": w2.Prop2DAcoVTIDenQ_DEO2_FDTD, "tti": w2.Prop2DAcoTTIDenQ_DEO2_FDTD}[kind]
+    prop = cls(nz=nz, nx=nx, dz=8.0, dx=10.0, dt=1e-3, nsponge=6, freesurface=fs, freqQ=10.0, qMin=0.5,
+               qInterior=80.0)
+    ocls = {"vti": o2.Oracle2DVTI, "tti": o2.Oracle2DTTI}[kind]
+    orc = ocls(nx, nz, 10.0, 8.0, 1e-3, fs=fs, nsponge=6, freqQ=10.0, qMin=0.5, qInterior=80.0)
+    arrs = make_model(kind, shape, seed)
+    rng = np.random.default_rng(seed + 1)
+    for k in ("pCur", "pOld", "mCur", "mOld"):
+        arrs[k] = smooth(rng, shape, 1.0, 2)
+    for name, a in arrs.items():
+        if name in ("sinPhi", "cosPhi"):
+            continue
+        getattr(w2, _GET[name])(prop)[...] = a.T
+        getattr(orc, name)[...] = a
+    return prop, orc
+
+
+ANISO_SHAPES = [(37, 45), (21, 70)]
+
+
+@pytest.mark.parametrize("kind", ["vti", "tti"])
+@pytest.mark.parametrize("fs", [False, True])
+@pytest.mark.parametrize("shape", ANISO_SHAPES)
+def test_aniso_steps_bit_exact(kind, fs, shape):
+    import wavefd_amd.prop2d as w2
+    prop, orc = _apair(kind, shape, fs, 21)
+    assert np.array_equal(_get(prop, "DtOmegaInvQ"), orc.q)
+    for _ in range(6):
+        w2.propagateforward(prop)
+        orc.time_step()
+    for name in ("pCur", "pOld", "mCur", "mOld", "pSpace", "mSpace"):
+        assert np.array_equal(_get(prop, _GET[name]), getattr(orc, name)), name
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_vti_linear_step_bit_exact(fs):
+    import wavefd_amd.prop2d as w2
+    prop, orc = _apair("vti", (37, 45), fs, 23)
+    for _ in range(4):
+        w2.propagateforward_linear(prop)
+        orc.time_step_linear()
+    for name in ("pCur", "pOld", "mCur", "mOld", "pSpace"):
+        assert np.array_equal(_get(prop, _GET[name]), getattr(orc, name)), name
+
+
+@pytest.mark.parametrize("kind", ["vti", "tti"])
+def test_aniso_many_steps(kind):
+    import wavefd_amd.prop2d as w2
+    prop, orc = _apair(kind, (48, 52), True, 25)
+    for _ in range(300):
+        w2.propagateforward(prop)
+        orc.time_step()
+    assert np.array_equal(_get(prop, "PCur"), orc.pCur)
+    assert np.array_equal(_get(prop, "MCur"), orc.mCur)
+
+
+def _amodel(w2, kind, which):
+    pre = {"vti": "Prop2DAcoVTIDenQ_DEO2_FDTD", "tti": "Prop2DAcoTTIDenQ_DEO2_FDTD"}[kind]
+    return getattr(w2, pre + "_Model_" + which)()
+
+
+@pytest.mark.parametrize("kind", ["vti", "tti"])
+@pytest.mark.parametrize("which", ["V", "VEA"])
+@pytest.mark.parametrize("fs", [False, True])
+def test_aniso_born_source_bit_exact(kind, which, fs):
+    import wavefd_amd.prop2d as w2
+    shape = (37, 45)
+    prop, orc = _apair(kind, shape, fs, 27)
+    rng = np.random.default_rng(28)
+    dv, de, da = smooth(rng, shape, 50.0, 2), smooth(rng, shape, 0.02, 2), smooth(rng, shape, 0.02, 2)
+    P, M, DP, DM = (smooth(rng, shape, 1.0, 2) for _ in range(4))
+    w2.forwardBornInjection(prop, _amodel(w2, kind, which), {"v": dv, "ϵ": de, "η": da},
+                            {"pold": P, "mold": M, "pspace": DP, "mspace": DM})
+    if which == "V":
+        orc.forward_born_injection_V(dv, DP, DM)
+    else:
+        orc.forward_born_injection_VEA(dv, de, da, P, M, DP, DM)
+    assert np.array_equal(_get(prop, "PCur"), orc.pCur)
+    assert np.array_equal(_get(prop, "MCur"), orc.mCur)
+
+
+@pytest.mark.parametrize("kind", ["vti", "tti"])
+@pytest.mark.parametrize("which", ["V", "VEA"])
+@pytest.mark.parametrize("fs", [False, True])
+def test_aniso_imaging_bit_exact(kind, which, fs):
+    import wavefd_amd.prop2d as w2
+    shape = (37, 45)
+    prop, orc = _apair(kind, shape, fs, 29)
+    rng = np.random.default_rng(30)
+    P, M, DP, DM = (smooth(rng, shape, 1.0, 2) for _ in range(4))
+    g = {k: smooth(rng, shape, 1.0, 2) for k in ("v", "ϵ", "η")}
+    g2 = {k: a.copy() for k, a in g.items()}
+    w2.adjointBornAccumulation(prop, _amodel(w2, kind, which), w2.ImagingConditionStandard(), g,
+                               {"pold": P, "mold": M, "pspace": DP, "mspace": DM})
+    if which == "V":
+        orc.adjoint_born_accumulation_V(g2["v"], DP, DM)
+    else:
+        orc.adjoint_born_accumulation_VEA(g2["v"], g2["ϵ"], g2["η"], P, M, DP, DM)
+    for k in g:
+        assert np.array_equal(g[k], g2[k]), k
+
+
+@pytest.mark.parametrize("kind", ["vti", "tti"])
+@pytest.mark.parametrize("fwi", [True, False])
+def test_aniso_wavefield_separation(kind, fwi):
+    from oracle.oracle import wavefield_separation
+    import wavefd_amd.prop2d as w2
+    shape = (23, 40)
+    prop, orc = _apair(kind, shape, True, 31)
+    rng = np.random.default_rng(32)
+    DP, DM = smooth(rng, shape, 1.0, 2), smooth(rng, shape, 1.0, 2)
+    g = np.zeros(shape, F32)
+    ic = w2.ImagingConditionWaveFieldSeparationFWI() if fwi else w2.ImagingConditionWaveFieldSeparationRTM()
+    w2.adjointBornAccumulation(prop, _amodel(w2, kind, "V"), ic, {"v": g}, {"pspace": DP, "mspace": DM})
+    # float64 restatement of src/prop2DAcoVTIDenQ_DEO2_FDTD.h:509-627 (FFTW's float32 bits are
+    # not available): per x column, the nonlinear and adjoint P and M parts summed
+    s = (wavefield_separation(DP[:, None, :], orc.pOld[:, None, :], fwi) +
+         wavefield_separation(DM[:, None, :], orc.mOld[:, None, :], fwi))[:, 0, :]
+    want = 2 * orc.b.astype(np.float64) / orc.v.astype(np.float64) ** 3 * s
+    err = np.linalg.norm(g - want) / np.linalg.norm(want)
+    assert err <= 2e-5, err
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_apply_first_derivatives_host_bit_exact(fs):
+    import wavefd_amd.prop2d as w2
+    nx, nz = 29, 33
+    rng = np.random.default_rng(33)
+    a, c = smooth(rng, (nx, nz), 1.0, 2), smooth(rng, (nx, nz), 1.0, 2)
+    ox, oz = np.full((nx, nz), 9.0, F32), np.full((nx, nz), 9.0, F32)
+    c8 = (1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0)
+    idx, idz = F32(1.0 / F32(10.0)), F32(1.0 / F32(8.0))
+    w2.applyFirstDerivatives2D_PlusHalf(fs, nx, nz, c8, idx, idz, a, c, ox, oz)
+    wx, _ = o2.dplus(o2.Grid2(nx, nz, 10.0, 8.0, fs), a, a)
+    _, wz = o2.dplus(o2.Grid2(nx, nz, 10.0, 8.0, fs), c, c)
+    assert np.array_equal(ox, wx)
+    assert np.array_equal(oz, wz)

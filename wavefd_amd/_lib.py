@@ -17,6 +17,8 @@ PREFIXES = {
     "vti": "Prop3DAcoVTIDenQ_DEO2_FDTD",
     "tti": "Prop3DAcoTTIDenQ_DEO2_FDTD",
     "iso2d": "Prop2DAcoIsoDenQ_DEO2_FDTD",
+    "vti2d": "Prop2DAcoVTIDenQ_DEO2_FDTD",
+    "tti2d": "Prop2DAcoTTIDenQ_DEO2_FDTD",
 }
 
 # name -> (restype, argtypes); common to the three propagator libraries
@@ -176,6 +178,45 @@ _ISO2D = {
     "ClearError": (None, []),
 }
 
+# the 2D VTI and TTI libraries (src/prop2DAcoVTIDenQ_DEO2_FDTD.cc, src/prop2DAcoTTIDenQ_DEO2_FDTD.cc)
+_ANISO2D = {
+    "alloc": (_P, [_L, _L, _L, _L, _L, _F, _F, _F, _L, _L]),
+    "free": (None, [_P]),
+    "SetupDtOmegaInvQ": (None, [_P, _F, _F, _F]),
+    "TimeStep": (None, [_P]),
+    "ScaleSpatialDerivatives": (None, [_P]),
+    "ForwardBornInjection_V": (None, [_P, _FP, _FP, _FP]),
+    "ForwardBornInjection_VEA": (None, [_P, _FP, _FP, _FP, _FP, _FP, _FP, _FP]),
+    "AdjointBornAccumulation_V": (None, [_P, _FP, _FP, _FP]),
+    "AdjointBornAccumulation_wavefieldsep_V": (None, [_P, _FP, _FP, _FP, _L]),
+    "AdjointBornAccumulation_VEA": (None, [_P, _FP, _FP, _FP, _FP, _FP, _FP, _FP]),
+    "getNx": (_L, [_P]),
+    "getNz": (_L, [_P]),
+    "getV": (_P, [_P]),
+    "getEps": (_P, [_P]),
+    "getEta": (_P, [_P]),
+    "getB": (_P, [_P]),
+    "getF": (_P, [_P]),
+    "getDtOmegaInvQ": (_P, [_P]),
+    "getPSpace": (_P, [_P]),
+    "getMSpace": (_P, [_P]),
+    "getPCur": (_P, [_P]),
+    "getPOld": (_P, [_P]),
+    "getMCur": (_P, [_P]),
+    "getMOld": (_P, [_P]),
+    "SetHostCoherence": (None, [_P, _L]),
+    "SyncToDevice": (None, [_P]),
+    "SyncToHost": (None, [_P]),
+    "GetDevicePtr": (_P, [_P, ctypes.c_char_p]),
+    "LastError": (ctypes.c_char_p, []),
+    "ClearError": (None, []),
+}
+_VTI2D = dict(_ANISO2D, TimeStepLinear=(None, [_P]),
+              applyFirstDerivatives2D_PlusHalf=(None, [_L, _L, _L, _L, _F, _F, _F, _F, _F, _F, _FP, _FP, _FP, _FP,
+                                                       _L, _L]))
+_TTI2D = dict(_ANISO2D, getSinTheta=(_P, [_P]), getCosTheta=(_P, [_P]))
+_2D = {"iso2d": _ISO2D, "vti2d": _VTI2D, "tti2d": _TTI2D}
+
 _TABLES = {"spacetime": SPACETIME, "codec": CODEC, "ginsu": GINSU, "lcodec": LCODEC}
 _LIBNAMES = {"spacetime": "libspacetime.so", "codec": "libwavefd_codec.so", "ginsu": "libwavefd_ginsu.so",
              "lcodec": "libwavefd_lcodec.so"}
@@ -189,8 +230,8 @@ def signatures(kind):
     if kind in _TABLES:
         return dict(_TABLES[kind])
     pre = PREFIXES[kind]
-    if kind == "iso2d":
-        return {pre + "_" + name: sig for name, sig in _ISO2D.items()}
+    if kind in _2D:
+        return {pre + "_" + name: sig for name, sig in _2D[kind].items()}
     out = {}
     for name, sig in list(_COMMON.items()) + list(_PHYS[kind].items()):
         out[pre + "_" + name] = sig

@@ -19,6 +19,8 @@ LIBS = {
     "libprop3DAcoVTIDenQ_DEO2_FDTD.so": ("prop3DAcoVTIDenQ_DEO2_FDTD.hip", ["-lhipfft"]),
     "libprop3DAcoTTIDenQ_DEO2_FDTD.so": ("prop3DAcoTTIDenQ_DEO2_FDTD.hip", ["-lhipfft"]),
     "libprop2DAcoIsoDenQ_DEO2_FDTD.so": ("prop2DAcoIsoDenQ_DEO2_FDTD.hip", ["-lhipfft"]),
+    "libprop2DAcoVTIDenQ_DEO2_FDTD.so": ("prop2DAcoVTIDenQ_DEO2_FDTD.hip", ["-lhipfft"]),
+    "libprop2DAcoTTIDenQ_DEO2_FDTD.so": ("prop2DAcoTTIDenQ_DEO2_FDTD.hip", ["-lhipfft"]),
     "libspacetime.so": ("spacetime.hip", []),
     # snapshot codec behind the Compressor's lossy mode (src/compressedio.jl); additive
     "libwavefd_codec.so": ("codec.hip", []),

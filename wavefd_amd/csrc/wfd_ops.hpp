@@ -647,4 +647,108 @@ static inline bool iso2_acc(Prop *p, float *gV, float *gB, float *P, float *DP) 
     return p->end(0);
 }
 
+// 2D VTI/TTI VEA source (src/prop2DAcoVTIDenQ_DEO2_FDTD.h:383-462,
+// src/prop2DAcoTTIDenQ_DEO2_FDTD.h:362-441): D+ (rotated for TTI) of P and M, the perturbation
+// sandwich, D- (rotated for TTI), then p += dt^2 v^2/b (2b dV/v^3 DP + lx + lz), m likewise
+static inline bool aniso2_inj_vea(Prop *p, float *dV, float *dE, float *dA, float *P, float *M, float *DP,
+                                  float *DM) {
+    if (!p->begin(p->model_mask() | bit(F_PCUR) | bit(F_MCUR))) return false;
+    Prop::Arg a[7];
+    WFD_ARG(dV, 0, true, false, a[0]);
+    WFD_ARG(dE, 1, true, false, a[1]);
+    WFD_ARG(dA, 2, true, false, a[2]);
+    WFD_ARG(P, 3, true, false, a[3]);
+    WFD_ARG(M, 4, true, false, a[4]);
+    WFD_ARG(DP, 5, true, false, a[5]);
+    WFD_ARG(DM, 6, true, false, a[6]);
+    float *t[8];
+    for (int i = 0; i < 8; i++)
+        if (!(t[i] = p->scratch(i))) return false;
+    if (!aniso2_dplus(p, a[3].dev, t[0], t[1])) return false;
+    if (!aniso2_dplus(p, a[4].dev, t[2], t[3])) return false;
+    k2_vea_inj_mid<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->f[F_B].d, p->f[F_ETA].d, p->f[F_F].d,
+                                                              a[1].dev, a[2].dev, t[0], t[1], t[3], t[4], t[5], t[6],
+                                                              t[7]);
+    WFD_LAUNCHED();
+    if (!aniso2_dminus(p, t[4], t[5], t[0], t[1])) return false;
+    if (!aniso2_dminus(p, t[6], t[7], t[2], t[3])) return false;
+    k2_vea_inj_final<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, p->f[F_V].d, p->f[F_B].d, a[0].dev,
+                                                                a[5].dev, a[6].dev, t[0], t[1], t[2], t[3],
+                                                                p->f[F_PCUR].d, p->f[F_MCUR].d);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, 7)) return false;
+    return p->end(bit(F_PCUR) | bit(F_MCUR));
+}
+
+// 2D VTI/TTI VEA imaging (src/prop2DAcoVTIDenQ_DEO2_FDTD.h:629-686,
+// src/prop2DAcoTTIDenQ_DEO2_FDTD.h:607-666)
+static inline bool aniso2_acc_vea(Prop *p, float *gV, float *gE, float *gA, float *P, float *M, float *DP,
+                                  float *DM) {
+    if (!p->begin(p->model_mask() | bit(F_POLD) | bit(F_MOLD))) return false;
+    Prop::Arg a[7];
+    WFD_ARG(gV, 0, true, true, a[0]);
+    WFD_ARG(gE, 1, true, true, a[1]);
+    WFD_ARG(gA, 2, true, true, a[2]);
+    WFD_ARG(P, 3, true, false, a[3]);
+    WFD_ARG(M, 4, true, false, a[4]);
+    WFD_ARG(DP, 5, true, false, a[5]);
+    WFD_ARG(DM, 6, true, false, a[6]);
+    float *t[8];
+    for (int i = 0; i < 8; i++)
+        if (!(t[i] = p->scratch(i))) return false;
+    if (!aniso2_dplus(p, a[3].dev, t[0], t[1])) return false;
+    if (!aniso2_dplus(p, a[4].dev, t[2], t[3])) return false;
+    if (!aniso2_dplus(p, p->f[F_POLD].d, t[4], t[5])) return false;
+    if (!aniso2_dplus(p, p->f[F_MOLD].d, t[6], t[7])) return false;
+    k2_vea_acc<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(
+        p->n, p->f[F_V].d, p->f[F_B].d, p->f[F_ETA].d, p->f[F_F].d, a[5].dev, a[6].dev, p->f[F_POLD].d,
+        p->f[F_MOLD].d, t[0], t[1], t[3], t[4], t[5], t[7], a[0].dev, a[1].dev, a[2].dev);
+    WFD_LAUNCHED();
+    if (!p->finish_args(a, 7)) return false;
+    return p->end(0);
+}
+
+// Prop2DAcoVTIDenQ_DEO2_FDTD_applyFirstDerivatives2D_PlusHalf: the stateless D+ on host arrays
+// with the caller's coefficients (src/prop2DAcoVTIDenQ_DEO2_FDTD.cc:154-175); staged through
+// device buffers on a stream of its own, synchronous like the reference
+static inline bool dplus_2d_host(long fs, long nx, long nz, float c1, float c2, float c3, float c4, float invDx,
+                                 float invDz, const float *inX, const float *inZ, float *outX, float *outZ) {
+    if (nx <= 0 || nz <= 0 || !inX || !inZ || !outX || !outZ) {
+        set_error("applyFirstDerivatives2D_PlusHalf: empty grid or null array");
+        return false;
+    }
+    const size_t bytes = sizeof(float) * (size_t)nx * (size_t)nz;
+    Grid g{};
+    g.nx = nx;
+    g.ny = 1;
+    g.nz = nz;
+    g.nynz = nz;
+    g.fs = fs ? 1 : 0;
+    g.idx = invDx;
+    g.idy = invDx;
+    g.idz = invDz;
+    hipStream_t st = nullptr;
+    float *d[4] = {};
+    bool ok = false;
+    do {
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) break;
+        bool alloc = true;
+        for (auto &x : d) alloc = alloc && hipMalloc(&x, bytes) == hipSuccess;
+        if (!alloc) break;
+        if (hipMemcpyAsync(d[0], inX, bytes, hipMemcpyHostToDevice, st) != hipSuccess) break;
+        if (hipMemcpyAsync(d[1], inZ, bytes, hipMemcpyHostToDevice, st) != hipSuccess) break;
+        k2_dplus_coef<<<op2_grid(g), op2_block(), 0, st>>>(g, make_float4(c1, c2, c3, c4), d[0], d[1], d[2], d[3]);
+        if (hipGetLastError() != hipSuccess) break;
+        if (hipMemcpyAsync(outX, d[2], bytes, hipMemcpyDeviceToHost, st) != hipSuccess) break;
+        if (hipMemcpyAsync(outZ, d[3], bytes, hipMemcpyDeviceToHost, st) != hipSuccess) break;
+        ok = hipStreamSynchronize(st) == hipSuccess;
+    } while (false);
+    if (!ok) set_error(std::string("applyFirstDerivatives2D_PlusHalf: ") + hipGetErrorString(hipGetLastError()));
+    for (auto &x : d)
+        if (x) (void)hipFree(x);
+    if (st) (void)hipStreamDestroy(st);
+    return ok;
+}
+
 }  // namespace wfd
+

@@ -191,7 +191,8 @@ struct Prop {
     Mask model_mask() const {
         Mask m = bit(F_V) | bit(F_B) | bit(F_Q);
         if (phys != ISO) m |= bit(F_EPS) | bit(F_ETA) | bit(F_F);
-        if (phys == TTI) m |= bit(F_ST) | bit(F_CT) | bit(F_SP) | bit(F_CP);
+        if (phys == TTI) m |= bit(F_ST) | bit(F_CT);
+        if (phys == TTI && dim == 3) m |= bit(F_SP) | bit(F_CP);  // the 2D TTI has no azimuth
         return m;
     }
     Mask wave_mask() const {
@@ -346,7 +347,7 @@ struct Prop {
 // ---- common operations ---------------------------------------------------------------
 
 static inline Prop *make_prop(Physics phys, long fs, long nthread, long nx, long ny, long nz, long nsponge, float dx,
-                              float dy, float dz, float dt, long nbx, long nby, long nbz) {
+                              float dy, float dz, float dt, long nbx, long nby, long nbz, int dim = 3) {
     if (nx <= 0 || ny <= 0 || nz <= 0 || nsponge <= 0) {
         set_error("alloc: nx, ny, nz and nsponge must be positive");
         return nullptr;
@@ -358,6 +359,7 @@ static inline Prop *make_prop(Physics phys, long fs, long nthread, long nx, long
     }
     Prop *p = new Prop();
     p->phys = phys;
+    p->dim = dim;
     p->fs = fs > 0;
     p->nthread = nthread;
     p->nx = nx; p->ny = ny; p->nz = nz;
