@@ -94,7 +94,7 @@ def build_model(torch, physics, n, d, dt, seed, dev):
     return m
 
 
-def run_gpu(physics, n, steps, warmup, seed=3):
+def run_gpu(physics, n, steps, warmup, seed=3, tune=False):
     import torch
     import wavefd_amd as w
 
@@ -130,6 +130,13 @@ def run_gpu(physics, n, steps, warmup, seed=3):
     torch.cuda.synchronize()
     setup_s = time.time() - t0
 
+    tuned = None
+    if tune:
+        # the launch-configuration autotuner (setup, before the timed region; it restores the
+        # wavefields it steps)
+        best, table = w.autotune(prop, nsteps=4, warmup=2)
+        tuned = {"best": best, "candidates": [list(r) for r in table]}
+    launch = w.fused_config(prop)
     stream = torch.cuda.ExternalStream(prop._lib.GetStream(prop.p))
     for _ in range(warmup):
         w.propagateforward(prop)
@@ -160,7 +167,7 @@ def run_gpu(physics, n, steps, warmup, seed=3):
         "ms_per_step": ms_per_step, "kernel_ms": kernel_ms,
         "mcells_s": cells * steps / wall / 1e6,
         "achieved_gbs": cells * BYTES_PER_CELL[physics] / (kernel_ms * 1e-3) / 1e9,
-        "finite": finite,
+        "finite": finite, "launch": launch, "autotune": tuned,
     }
 
 
@@ -533,6 +540,8 @@ def main():
     ap.add_argument("--codec", action="store_true", help="only train and measure the learned snapshot codec")
     ap.add_argument("--coherent", action="store_true",
                     help="also time the reference's host loop through the coherent API (PCIe-inclusive)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the launch-configuration autotuner (the automatic rule then chooses)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel 3 times (the known-size "
                          "kernel that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
@@ -552,7 +561,7 @@ def main():
               "build has no multi-GPU path; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    res = run_gpu(args.physics, args.n, args.steps, args.warmup)
+    res = run_gpu(args.physics, args.n, args.steps, args.warmup, tune=not args.no_autotune)
     extra = []
     if args.extra:
         for ph, n in (("iso", 256), ("vti", 512)):
@@ -592,9 +601,12 @@ def main():
             "traffic": measured_traffic(args.physics, args.n),
             "bytes_per_cell": BYTES_PER_CELL[args.physics],
             "kernel_ms": round(res["kernel_ms"], 4),
+            "launch": res["launch"],
         },
         "cpu_baseline": cpu,
     }
+    if res["autotune"]:
+        line["autotune"] = res["autotune"]
     if extra:
         line["extra"] = extra
     print(json.dumps(line), flush=True)

@@ -75,6 +75,13 @@ struct SourcePoint32; /* spacetime.h */
      * results do not depend on it.  0 on success, -1 for another value. */                         \
     long P##_SetFusedTile(void *p, long tile);                                                      \
     long P##_GetFusedTile(void *p);                                                                 \
+    /* the fused step's whole launch configuration: tile as above, rounds in step (lockstep)       \
+     * -1 automatic / 0 off / 1 on (only where the grid has 16 rounds of workgroups), x-planes per  \
+     * chunk (0 automatic); results do not depend on it.  0 on success.  GetFusedConfig writes     \
+     * what a plain TimeStep launches with: out[0] tile, out[1] lockstep, out[2] chunk,             \
+     * out[3] workgroups.  The autotuner (wavefd_amd/tuning.py) sets it.  */                      \
+    long P##_SetFusedConfig(void *p, long tile, long lockstep, long chunk);                         \
+    long P##_GetFusedConfig(void *p, long *out);                                                    \
     /* step n (1..8) distinct handles of this physics, all in DEVICE mode, by one launch: the     \
      * same results as calling on each handle i TimeStep (epilogue 0), TimeStepSaveDP (1: a0[i] =  \
      * DP, a1[i] = DM), TimeStepBornInjection_V (2: a0 = dV, a1 = DP, a2 = DM) or                  \

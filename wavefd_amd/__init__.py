@@ -1,8 +1,9 @@
-"""wavefd_amd -- MI355X (gfx950) implementation of WaveFD's 3D acoustic propagator path.
+"""wavefd_amd -- MI355X (gfx950) implementation of WaveFD's acoustic propagator path.
 
-The compute lives in four in-tree HIP libraries with the reference's names and C-ABI
-(wavefd_amd/lib/lib{prop3DAco{Iso,VTI,TTI}DenQ_DEO2_FDTD,spacetime}.so); this package is the
-host-side mirror of the reference's Julia API over them.
+The compute lives in in-tree HIP libraries with the reference's names and C-ABI
+(wavefd_amd/lib/lib{prop3DAco{Iso,VTI,TTI}DenQ_DEO2_FDTD,prop2DAco{Iso,VTI,TTI}DenQ_DEO2_FDTD,
+spacetime}.so); this package is the host-side mirror of the reference's Julia API over them
+(the 2D propagators in wavefd_amd.prop2d).
 """
 from .prop3d import (  # noqa: F401
     B, CosPhi, CosTheta, DtOmegaInvQ, Eps, Eta, F, ImagingConditionStandard,
@@ -18,3 +19,4 @@ from .prop3d import (  # noqa: F401
 from .spacetime import (  # noqa: F401
     SourcePoint32, causal_ricker, extractdata, hickscoeffs, injectdata, interpadjoint, interpfilters,
     interpforward, linearcoeffs, receiver_blocking, ricker, source_blocking)
+from .tuning import autotune, fused_config, set_fused_config, tilesize  # noqa: F401,E402

@@ -50,6 +50,8 @@ _COMMON = {
     "SetFusedStep": (None, [_P, _L]),
     "SetFusedTile": (_L, [_P, _L]),
     "GetFusedTile": (_L, [_P]),
+    "SetFusedConfig": (_L, [_P, _L, _L, _L]),
+    "GetFusedConfig": (_L, [_P, _P]),
     "TimeStepBatch": (_L, [_P, _L, _L, _P, _P, _P]),
     "LastError": (ctypes.c_char_p, []),
     "ClearError": (None, []),

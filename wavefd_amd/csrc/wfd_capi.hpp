@@ -107,6 +107,12 @@
         return wfd::set_fused_tile(reinterpret_cast<wfd::Prop *>(p), tile) ? 0 : -1;                      \
     }                                                                                                     \
     long WFD_CAT(Prefix, GetFusedTile)(void *p) { return reinterpret_cast<wfd::Prop *>(p)->fused_tile; }  \
+    long WFD_CAT(Prefix, SetFusedConfig)(void *p, long tile, long lockstep, long chunk) {                 \
+        return wfd::set_fused_config(reinterpret_cast<wfd::Prop *>(p), tile, lockstep, chunk) ? 0 : -1;   \
+    }                                                                                                     \
+    long WFD_CAT(Prefix, GetFusedConfig)(void *p, long *out) {                                            \
+        return wfd::get_fused_config(reinterpret_cast<wfd::Prop *>(p), out) ? 0 : -1;                     \
+    }                                                                                                     \
     long WFD_CAT(Prefix, TimeStepBatch)(void **ps, long n, long epilogue, float **a0, float **a1, float **a2) { \
         return wfd::time_step_batch(PHYS, ps, n, epilogue, a0, a1, a2) ? 0 : -1;                          \
     }                                                                                                     \

@@ -545,6 +545,7 @@ __global__ void __launch_bounds__(ZW *NROW) k_fused_batch(FusedBatch<PHYS> B) {
 // x-planes per chunk: balance whole rounds of workgroups (one per CU) against the 7-step
 // warm-up and the padding to a multiple of 4 steps that each chunk pays
 static inline int fused_cx(const Prop *p, long tiles) {
+    if (p->fused_chunk > 0) return (int)std::min<long>(p->fused_chunk, p->nx);
     if (const char *e = std::getenv("WAVEFD_FUSED_CX")) {
         const long v = std::atol(e);
         if (v > 0) return (int)std::min<long>(v, p->nx);
@@ -623,6 +624,18 @@ static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float 
     return (long)A.nzt * A.nyt * nxc;
 }
 
+// rounds in step: the automatic rule, overridden by the handle's setting, else by the
+// environment; never where the rounds cannot be formed
+static inline bool fused_lockstep_of(const Prop *p, int phys, int epi, long ntiles) {
+    const bool can = p->cu_count % 8 == 0 && ntiles >= 16L * p->cu_count;
+    bool lockstep = can && phys != ISO && epi == EPI_NONE;
+    if (p->fused_lockstep >= 0)
+        lockstep = can && p->fused_lockstep != 0;
+    else if (const char *e = std::getenv("WAVEFD_FUSED_LOCKSTEP"))
+        lockstep = can && std::atoi(e) != 0;
+    return lockstep;
+}
+
 template <int PHYS, int ZW, int NROW, int EPI>
 static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float *e1 = nullptr,
                                 float *e2 = nullptr) {
@@ -635,9 +648,7 @@ static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float
     // 6876 against 7032 shots per hour), the waiting costs more than the shared lines save.  It
     // needs many rounds: 800^3 (21 rounds) TTI 13.31 -> 12.73 ms, VTI 10.36 -> 10.14 ms; 512^3
     // (9 rounds) TTI 3.43 -> 3.70 ms, VTI 2.79 -> 2.99 ms.
-    const bool can = p->cu_count % 8 == 0 && ntiles >= 16L * p->cu_count;
-    bool lockstep = can && PHYS != ISO && EPI == EPI_NONE;
-    if (const char *e = std::getenv("WAVEFD_FUSED_LOCKSTEP")) lockstep = can && std::atoi(e) != 0;
+    const bool lockstep = fused_lockstep_of(p, PHYS, EPI, ntiles);
     if (lockstep && p->fused_sync) {
         A.per_xcd = p->cu_count / 8;
         WFD_TRY(hipMemsetAsync(p->fused_sync, 0, 8 * sizeof(unsigned), p->stream));
@@ -676,6 +687,38 @@ static inline int fused_tile_of(const Prop *p, int epi) {
     // the Born epilogues read 1 to 3 more fields per output: the wide tile wins there (Iso Born
     // passes on 200^3..600^3 windows: 6798 against 6416 shots per hour)
     return p->phys == ISO && epi == EPI_NONE ? TILE_32x32 : TILE_64x16;
+}
+
+// the fused step's launch configuration (tile, rounds in step, x-chunk length); results do not
+// depend on it.  tile 0..2 as SetFusedTile, lockstep -1 automatic / 0 / 1, chunk 0 automatic.
+static inline bool set_fused_config(Prop *p, long tile, long lockstep, long chunk) {
+    if (lockstep < -1 || lockstep > 1 || chunk < 0) {
+        set_error("SetFusedConfig: lockstep must be -1, 0 or 1 and chunk >= 0");
+        return false;
+    }
+    if (!set_fused_tile(p, tile)) return false;
+    p->fused_lockstep = (int)lockstep;
+    p->fused_chunk = chunk;
+    return true;
+}
+
+// the configuration a plain TimeStep launches with: out[0] tile (1 or 2), out[1] lockstep
+// (0 or 1), out[2] x-planes per chunk, out[3] workgroups
+static inline bool get_fused_config(const Prop *p, long *out) {
+    if (!out) {
+        set_error("GetFusedConfig: null output");
+        return false;
+    }
+    const int tile = fused_tile_of(p, EPI_NONE);
+    const int zo = (tile == TILE_32x32 ? 32 : 64) - 7, yo = (tile == TILE_32x32 ? 32 : 16) - 7;
+    const long tiles = ((p->nz + zo - 1) / zo) * ((p->ny + yo - 1) / yo);
+    const long cx = fused_cx(p, tiles);
+    const long ntiles = tiles * ((p->nx + cx - 1) / cx);
+    out[0] = tile;
+    out[1] = fused_lockstep_of(p, p->phys, EPI_NONE, ntiles) ? 1 : 0;
+    out[2] = cx;
+    out[3] = ntiles;
+    return true;
 }
 
 template <int PHYS, int EPI>

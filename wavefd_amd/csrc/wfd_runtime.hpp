@@ -182,6 +182,8 @@ struct Prop {
     int src_fields = 1, rcv_fields = 1;
     int fused = 1;  // use the fused single-pass step kernels
     int fused_tile = 0;  // fused kernel tile: 0 automatic, 1 64 (z) x 16 (y), 2 32 x 32
+    int fused_lockstep = -1;  // rounds in step: -1 automatic, 0 off, 1 on (where it can run)
+    long fused_chunk = 0;     // x-planes per chunk: 0 automatic
     int cu_count = 256;  // compute units of the device (sizes the fused kernel's x-chunks)
     unsigned *fused_sync = nullptr;  // per-XCD round counters of the persistent fused launch
     hipEvent_t ev = nullptr;         // orders this handle's stream against a batch's (lazy)
