@@ -171,6 +171,61 @@ def run_gpu(physics, n, steps, warmup, seed=3, tune=False):
     }
 
 
+# 2D steps: compulsory HBM bytes per cell (Iso: pCur, pOld, v, b, Q in, p+, pSpace out; VTI adds
+# mCur, mOld, eps, eta, f in and m+, mSpace out; TTI adds sin/cos theta) and the bytes the two
+# passes move (the sandwich writes the temporaries, the update reads them back)
+BYTES_2D = {"iso": 28, "vti": 56, "tti": 64}
+MOVED_2D = {"iso": 48, "vti": 100, "tti": 116}
+
+
+def run_2d(physics, n, steps, warmup):
+    """2D forward step on an n x n grid at 12.5 m, dt 1 ms, free surface: HIP-event time per
+    step on the handle's stream, Mcells/s and HBM roofline fraction."""
+    import torch
+    import wavefd_amd.prop2d as w2
+
+    cls = {"iso": w2.Prop2DAcoIsoDenQ_DEO2_FDTD, "vti": w2.Prop2DAcoVTIDenQ_DEO2_FDTD,
+           "tti": w2.Prop2DAcoTTIDenQ_DEO2_FDTD}[physics]
+    prop = cls(nz=n, nx=n, dz=12.5, dx=12.5, dt=1e-3, nsponge=50, freesurface=True, freqQ=10.0, qMin=0.1,
+               qInterior=100.0)
+    rng = np.random.default_rng(5)
+    z = np.linspace(0.0, 1.0, n, dtype=np.float32)[:, None]
+    w2.V(prop)[...] = 1500 + 3000 * z + 50 * rng.standard_normal((n, n)).astype(np.float32)
+    w2.B(prop)[...] = 1 / (1.0 + 1.5 * z)
+    if physics != "iso":
+        w2.Eps(prop)[...] = 0.1
+        w2.Eta(prop)[...] = 0.05
+        w2.F(prop)[...] = 0.85
+    if physics == "tti":
+        th = np.deg2rad(30.0) * np.ones((n, n), np.float32)
+        w2.SinTheta(prop)[...] = np.sin(th)
+        w2.CosTheta(prop)[...] = np.cos(th)
+    for g in ((w2.PCur, w2.POld) + ((w2.MCur, w2.MOld) if physics != "iso" else ())):
+        g(prop)[...] = rng.standard_normal((n, n)).astype(np.float32) * 1e-3
+    w2.sync_to_device(prop)
+    w2.coherence(prop, "device")
+    stream = torch.cuda.ExternalStream(w2.stream(prop))
+    for _ in range(warmup):
+        w2.propagateforward(prop)
+    w2.synchronize(prop)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        w2.propagateforward(prop)
+    ev1.record(stream)
+    w2.synchronize(prop)
+    ms = ev0.elapsed_time(ev1) / steps
+    w2.free(prop)
+    cells = float(n) * n
+    gbs = cells * BYTES_2D[physics] / (ms * 1e-3) / 1e9
+    return {"physics": physics + "2d", "n": n, "steps": steps, "ms_per_step": round(ms, 4),
+            "mcells_s": round(cells / (ms * 1e-3) / 1e6, 1), "bytes_per_cell": BYTES_2D[physics],
+            "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "moved_bytes_per_cell": MOVED_2D[physics],
+            "moved_gbs": round(cells * MOVED_2D[physics] / (ms * 1e-3) / 1e9, 1)}
+
+
 def run_coherent(physics, n, steps, seed=3):
     """The reference's Julia loop (test/spacetime.jl:486-492) through the default COHERENT host
     API: the model filled through the getters' host arrays, then per step propagateforward
@@ -540,6 +595,8 @@ def main():
     ap.add_argument("--codec", action="store_true", help="only train and measure the learned snapshot codec")
     ap.add_argument("--coherent", action="store_true",
                     help="also time the reference's host loop through the coherent API (PCIe-inclusive)")
+    ap.add_argument("--prop2d", action="store_true",
+                    help="only time the 2D Iso, VTI and TTI steps (8192^2 cells)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the launch-configuration autotuner (the automatic rule then chooses)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -551,6 +608,9 @@ def main():
         return
     if args.codec:
         print(json.dumps({"codec": run_codec()}), flush=True)
+        return
+    if args.prop2d:
+        print(json.dumps({"prop2d": [run_2d(ph, 8192, 50, 5) for ph in ("iso", "vti", "tti")]}), flush=True)
         return
     if args.born:
         print(json.dumps({"born": run_born(args.born, args.born_shots, args.born_nt, fused=not args.born_plain,

@@ -68,6 +68,9 @@ void Prop2DAcoVTIDenQ_DEO2_FDTD_SetHostCoherence(void *p, long mode);
 void Prop2DAcoVTIDenQ_DEO2_FDTD_SyncToDevice(void *p);
 void Prop2DAcoVTIDenQ_DEO2_FDTD_SyncToHost(void *p);
 float *Prop2DAcoVTIDenQ_DEO2_FDTD_GetDevicePtr(void *p, const char *name);
+/* the handle's HIP stream (device-mode calls are ordered on it) and a wait for it */
+void *Prop2DAcoVTIDenQ_DEO2_FDTD_GetStream(void *p);
+void Prop2DAcoVTIDenQ_DEO2_FDTD_Synchronize(void *p);
 const char *Prop2DAcoVTIDenQ_DEO2_FDTD_LastError(void);
 void Prop2DAcoVTIDenQ_DEO2_FDTD_ClearError(void);
 

@@ -176,6 +176,8 @@ _ISO2D = {
     "SyncToDevice": (None, [_P]),
     "SyncToHost": (None, [_P]),
     "GetDevicePtr": (_P, [_P, ctypes.c_char_p]),
+    "GetStream": (_P, [_P]),
+    "Synchronize": (None, [_P]),
     "LastError": (ctypes.c_char_p, []),
     "ClearError": (None, []),
 }
@@ -210,6 +212,8 @@ _ANISO2D = {
     "SyncToDevice": (None, [_P]),
     "SyncToHost": (None, [_P]),
     "GetDevicePtr": (_P, [_P, ctypes.c_char_p]),
+    "GetStream": (_P, [_P]),
+    "Synchronize": (None, [_P]),
     "LastError": (ctypes.c_char_p, []),
     "ClearError": (None, []),
 }

@@ -85,6 +85,10 @@ float *Prop2DAcoTTIDenQ_DEO2_FDTD_GetDevicePtr(void *p, const char *name) {
     const int i = wfd::field_by_name(name);
     return (i >= 0 && P2(p)->f[i].used) ? P2(p)->f[i].d : nullptr;
 }
+void *Prop2DAcoTTIDenQ_DEO2_FDTD_GetStream(void *p) { return (void *)P2(p)->stream; }
+void Prop2DAcoTTIDenQ_DEO2_FDTD_Synchronize(void *p) {
+    if (hipStreamSynchronize(P2(p)->stream) != hipSuccess) wfd::set_error("Synchronize: stream error");
+}
 const char *Prop2DAcoTTIDenQ_DEO2_FDTD_LastError(void) { return wfd::g_last_error.c_str(); }
 void Prop2DAcoTTIDenQ_DEO2_FDTD_ClearError(void) { wfd::g_last_error.clear(); }
 

@@ -264,3 +264,40 @@ def applyFirstDerivatives2D_PlusHalf(freesurface, nx, nz, c8, invDx, invDz, inX,
     if msg:
         L.ClearError()
         raise RuntimeError(msg)
+
+
+# ---- host coherence / device path (additive) ------------------------------------------
+
+def coherence(prop, mode):
+    """Set the host-coherence mode: 'coherent' (default, the reference's synchronous host
+    arrays) or 'device' (calls stream-ordered on the handle's stream, host arrays untouched)."""
+    prop._lib.SetHostCoherence(prop.p, 1 if mode == "coherent" else 0)
+
+
+def sync_to_device(prop):
+    prop._lib.SyncToDevice(prop.p)
+    prop._check()
+
+
+def sync_to_host(prop):
+    prop._lib.SyncToHost(prop.p)
+    prop._check()
+
+
+def synchronize(prop):
+    prop._lib.Synchronize(prop.p)
+    prop._check()
+
+
+def stream(prop):
+    """The handle's HIP stream (an integer address)."""
+    return prop._lib.GetStream(prop.p)
+
+
+def device_ptr(prop, name):
+    """Device address of a state array by name (v b dtOmegaInvQ eps eta f sinTheta cosTheta
+    pCur pOld mCur mOld pSpace mSpace)."""
+    addr = prop._lib.GetDevicePtr(prop.p, name.encode())
+    if not addr:
+        raise KeyError(name)
+    return addr
