@@ -597,8 +597,10 @@ def main():
                     help="also time the reference's host loop through the coherent API (PCIe-inclusive)")
     ap.add_argument("--prop2d", action="store_true",
                     help="only time the 2D Iso, VTI and TTI steps (8192^2 cells)")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="skip the launch-configuration autotuner (the automatic rule then chooses)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="choose the fused step's launch configuration with the autotuner before timing "
+                         "(measured: it picks the automatic rule's configuration; off by default so that a "
+                         "kernel-trace of the default command holds only the timed configuration)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel 3 times (the known-size "
                          "kernel that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
@@ -621,7 +623,7 @@ def main():
               "build has no multi-GPU path; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    res = run_gpu(args.physics, args.n, args.steps, args.warmup, tune=not args.no_autotune)
+    res = run_gpu(args.physics, args.n, args.steps, args.warmup, tune=args.autotune)
     extra = []
     if args.extra:
         for ph, n in (("iso", 256), ("vti", 512)):
