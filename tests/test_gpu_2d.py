@@ -291,3 +291,22 @@ def test_apply_first_derivatives_host_bit_exact(fs):
     _, wz = o2.dplus(o2.Grid2(nx, nz, 10.0, 8.0, fs), c, c)
     assert np.array_equal(ox, wx)
     assert np.array_equal(oz, wz)
+
+
+@pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
+def test_2000_steps(kind):
+    # the north star's parity length: 2000 steps from a smooth state, identical to the oracle
+    import wavefd_amd.prop2d as w2
+    if kind == "iso":
+        prop, orc = _pair((40, 44), True, 41)
+    else:
+        prop, orc = _apair(kind, (40, 44), True, 41)
+    w2.sync_to_device(prop)  # the host-set state, before the calls stop uploading it
+    w2.coherence(prop, "device")
+    for _ in range(2000):
+        w2.propagateforward(prop)
+        orc.time_step()
+    w2.sync_to_host(prop)
+    assert np.array_equal(_get(prop, "PCur"), orc.pCur)
+    if kind != "iso":
+        assert np.array_equal(_get(prop, "MCur"), orc.mCur)

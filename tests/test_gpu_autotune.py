@@ -103,3 +103,11 @@ def test_autotune_restores_state_and_keeps_the_best(kind, mode):
     _, ref = _run(kind, shape, None, nsteps=1)
     assert np.array_equal(np.array(w.PCur(p)), ref[0])
     w.free(p)
+
+
+def test_tilesize_mirrors_the_reference_signature():
+    import wavefd_amd as w
+    rngx, rngy, speeds = w.tilesize(w.Prop3DAcoTTIDenQ_DEO2_FDTD, nz=40, ny=30, nx=36, rngy=[4], rngx=[4, 8], N=2)
+    assert list(rngx) == [4, 8] and list(rngy) == [4]
+    assert speeds.shape == (1, 2)
+    assert np.all(np.isfinite(speeds)) and np.all(speeds > 0)
