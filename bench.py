@@ -226,6 +226,63 @@ def run_2d(physics, n, steps, warmup):
             "moved_gbs": round(cells * MOVED_2D[physics] / (ms * 1e-3) / 1e9, 1)}
 
 
+def run_graph(physics="iso", n=96, nt=400, chunk=None):
+    """A small window's modelling loop (step, inject, extract per time step): the calls one by
+    one against graph segments of `chunk` steps (TimeStepInjectExtractGraph); wall time per step
+    with inputs resident on the GPU."""
+    import wavefd_amd as w
+    cls = {"iso": w.Prop3DAcoIsoDenQ_DEO2_FDTD, "vti": w.Prop3DAcoVTIDenQ_DEO2_FDTD,
+           "tti": w.Prop3DAcoTTIDenQ_DEO2_FDTD}[physics]
+    f32 = np.float32
+
+    def setup():
+        prop = cls(nz=n, ny=n, nx=n, dz=10.0, dy=10.0, dx=10.0, dt=1e-3, nsponge=10, freesurface=True)
+        w.V(prop)[...] = 2000
+        w.B(prop)[...] = 1
+        if physics != "iso":
+            w.Eps(prop)[...] = 0.1
+            w.Eta(prop)[...] = 0.05
+            w.F(prop)[...] = 0.85
+        if physics == "tti":
+            w.CosTheta(prop)[...] = 1
+            w.CosPhi(prop)[...] = 1
+        w.sync_to_device(prop)
+        w.coherence(prop, "device")
+        src = w.hickscoeffs(f32(10), f32(10), f32(10), 0.0, 0.0, 0.0, n, n, n, [100.0], [5.0 * n], [5.0 * n])
+        wavelet = w.causal_ricker(15.0, 1e-3 * np.arange(nt))[None, :]
+        r = np.linspace(100.0, 10.0 * (n - 10), 32)
+        rcv = w.linearcoeffs(f32(10), f32(10), f32(10), 0.0, 0.0, 0.0, n, n, n, np.full(32, 50.0), r, r)
+        w.set_sources(prop, w.source_blocking(src, 4), wavelet, fields=3 if physics != "iso" else 1)
+        w.set_receivers(prop, w.receiver_blocking(rcv, 4), nt)
+        return prop
+
+    prop = setup()
+    w.timestep_inject_extract(prop, 1)  # first-call costs outside the timing
+    w.synchronize(prop)
+    t0 = time.perf_counter()
+    for it in range(2, nt + 1):
+        w.timestep_inject_extract(prop, it)
+    w.synchronize(prop)
+    eager = (time.perf_counter() - t0) * 1e3 / (nt - 1)
+    d_eager = w.receiver_data(prop, 32, nt)
+    w.free(prop)
+    prop = setup()
+    w.timestep_inject_extract(prop, 1)
+    w.synchronize(prop)
+    t0 = time.perf_counter()
+    it = 2
+    chunk = chunk or nt
+    while it <= nt:
+        k = min(chunk, nt + 1 - it)
+        w.timestep_inject_extract_graph(prop, it, k)
+        it += k
+    graph = (time.perf_counter() - t0) * 1e3 / (nt - 1)
+    same = bool(np.array_equal(w.receiver_data(prop, 32, nt), d_eager))
+    w.free(prop)
+    return {"physics": physics, "n": n, "nt": nt, "chunk": chunk, "eager_ms_per_step": round(eager, 4),
+            "graph_ms_per_step": round(graph, 4), "speedup": round(eager / graph, 3), "traces_identical": same}
+
+
 def run_coherent(physics, n, steps, seed=3):
     """The reference's Julia loop (test/spacetime.jl:486-492) through the default COHERENT host
     API: the model filled through the getters' host arrays, then per step propagateforward
@@ -595,6 +652,8 @@ def main():
     ap.add_argument("--codec", action="store_true", help="only train and measure the learned snapshot codec")
     ap.add_argument("--coherent", action="store_true",
                     help="also time the reference's host loop through the coherent API (PCIe-inclusive)")
+    ap.add_argument("--graph", action="store_true",
+                    help="only time a small window's modelling loop, calls one by one against graph segments")
     ap.add_argument("--prop2d", action="store_true",
                     help="only time the 2D Iso, VTI and TTI steps (8192^2 cells)")
     ap.add_argument("--autotune", action="store_true",
@@ -610,6 +669,9 @@ def main():
         return
     if args.codec:
         print(json.dumps({"codec": run_codec()}), flush=True)
+        return
+    if args.graph:
+        print(json.dumps({"graph": [run_graph(ph, n) for ph in ("iso", "tti") for n in (64, 128)]}), flush=True)
         return
     if args.prop2d:
         print(json.dumps({"prop2d": [run_2d(ph, 8192, 50, 5) for ph in ("iso", "vti", "tti")]}), flush=True)

@@ -62,6 +62,12 @@ struct SourcePoint32; /* spacetime.h */
     /* TimeStep, then inject the sources at it (1-based) into the new pCur (mCur), then extract   \
      * the receivers at it from the new pCur: the loop body of test/spacetime.jl:486-492 */       \
     void P##_TimeStepInjectExtract(void *p, long it);                                               \
+    /* nsteps of TimeStepInjectExtract (it0 .. it0+nsteps-1) as a HIP graph of two steps, captured  \
+     * once on the handle's stream (the time index in a device counter the graph advances) and      \
+     * launched nsteps/2 times; the same results as the calls one by one.  DEVICE mode and the      \
+     * fused step only; time indices checked first; returns when the steps are done.  0 on          \
+     * success. */                                                                                  \
+    long P##_TimeStepInjectExtractGraph(void *p, long it0, long nsteps);                            \
     /* the two halves alone, for loops that put other work between the step and the             \
      * extraction (Born: step, Born source, extract; adjoint: step, adjoint source, image).       \
      * 0 on success. */                                                                             \

@@ -15,7 +15,7 @@ from .prop3d import (  # noqa: F401
     Prop3DAcoVTIDenQ_DEO2_FDTD_Model_VEA, SinPhi, SinTheta, V, adjointBornAccumulation, coherence,
     device_ptr, forwardBornInjection, free, propagateadjoint, propagateforward, propagateforward_linear,
     receiver_data, scale_spatial_derivatives, set_fused_step, set_fused_tile, set_receivers, set_receivers_interp, set_sources, set_sources_interp, size,
-    sync_to_device, sync_to_host, synchronize, timestep_inject_extract)
+    sync_to_device, sync_to_host, synchronize, timestep_inject_extract, timestep_inject_extract_graph)
 from .spacetime import (  # noqa: F401
     SourcePoint32, causal_ricker, extractdata, hickscoeffs, injectdata, interpadjoint, interpfilters,
     interpforward, linearcoeffs, receiver_blocking, ricker, source_blocking)

@@ -50,6 +50,7 @@ _COMMON = {
     "SetFusedStep": (None, [_P, _L]),
     "SetFusedTile": (_L, [_P, _L]),
     "GetFusedTile": (_L, [_P]),
+    "TimeStepInjectExtractGraph": (_L, [_P, _L, _L]),
     "SetFusedConfig": (_L, [_P, _L, _L, _L]),
     "GetFusedConfig": (_L, [_P, _P]),
     "TimeStepBatch": (_L, [_P, _L, _L, _P, _P, _P]),

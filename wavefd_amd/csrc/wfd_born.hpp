@@ -180,10 +180,13 @@ struct SourcePoint32 {
     float c;
 };
 
+// itp (optional): the time index is read from device memory instead of it (graph replays)
 __global__ void k_inject_groups(float *__restrict__ field, const float *__restrict__ data, long it, long nt,
-                                const SourcePoint32 *__restrict__ pts, const int *__restrict__ gstart, int ngroups) {
+                                const SourcePoint32 *__restrict__ pts, const int *__restrict__ gstart, int ngroups,
+                                const long *__restrict__ itp) {
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= ngroups) return;
+    if (itp) it = *itp;
     const int s = gstart[gi], e = gstart[gi + 1];
     const long iu = pts[s].iu - 1;
     float acc = field[iu];
@@ -192,15 +195,21 @@ __global__ void k_inject_groups(float *__restrict__ field, const float *__restri
 }
 
 __global__ void k_extract_groups(float *__restrict__ data, const float *__restrict__ field, long it, long nt,
-                                 const SourcePoint32 *__restrict__ pts, const int *__restrict__ gstart, int ngroups) {
+                                 const SourcePoint32 *__restrict__ pts, const int *__restrict__ gstart, int ngroups,
+                                 const long *__restrict__ itp) {
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= ngroups) return;
+    if (itp) it = *itp;
     const int s = gstart[gi], e = gstart[gi + 1];
     const long id = (pts[s].ir - 1) * nt + it - 1;
     float acc = data[id];
     for (int i = s; i < e; i++) acc += pts[i].c * field[pts[i].iu - 1];
     data[id] = acc;
 }
+
+// the device-side time index of a replayed graph
+__global__ void k_set_i64(long *p, long v) { *p = v; }
+__global__ void k_inc_i64(long *p) { *p += 1; }
 
 // ---- registered points with the time interpolation fused in -----------------------------
 // The reference's 8-tap polyphase interpolation (src/spacetime.cc:4-127) maps a coarse trace d
@@ -228,9 +237,10 @@ __device__ __forceinline__ bool interp_row(long jk, long nh, long nd, long nm, l
 __global__ void k_inject_groups_interp(float *__restrict__ field, const float *__restrict__ data, long it, long nd,
                                        const float *__restrict__ h, long nh, long nm,
                                        const SourcePoint32 *__restrict__ pts, const int *__restrict__ gstart,
-                                       int ngroups) {
+                                       int ngroups, const long *__restrict__ itp) {
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= ngroups) return;
+    if (itp) it = *itp;
     long i;
     int k, lo, hi;
     if (!interp_row(it - 1, nh, nd, nm, i, k, lo, hi)) return;  // the fine sample is 0: nothing to add
@@ -251,9 +261,10 @@ __global__ void k_inject_groups_interp(float *__restrict__ field, const float *_
 __global__ void k_extract_groups_interp(float *__restrict__ data, const float *__restrict__ field, long it, long nd,
                                         const float *__restrict__ h, long nh, long nm,
                                         const SourcePoint32 *__restrict__ pts, const int *__restrict__ gstart,
-                                        int ngroups) {
+                                        int ngroups, const long *__restrict__ itp) {
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= ngroups) return;
+    if (itp) it = *itp;
     long i;
     int k, lo, hi;
     if (!interp_row(it - 1, nh, nd, nm, i, k, lo, hi)) return;

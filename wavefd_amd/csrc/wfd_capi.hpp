@@ -93,6 +93,9 @@
     void WFD_CAT(Prefix, TimeStepInjectExtract)(void *p, long it) {                                       \
         wfd::time_step_inject_extract(reinterpret_cast<wfd::Prop *>(p), it, false);                       \
     }                                                                                                     \
+    long WFD_CAT(Prefix, TimeStepInjectExtractGraph)(void *p, long it0, long nsteps) {                   \
+        return wfd::time_loop_graph(reinterpret_cast<wfd::Prop *>(p), it0, nsteps) ? 0 : -1;             \
+    }                                                                                                     \
     long WFD_CAT(Prefix, InjectRegistered)(void *p, long it) {                                            \
         return wfd::inject_registered_call(reinterpret_cast<wfd::Prop *>(p), it) ? 0 : -1;                \
     }                                                                                                     \

@@ -483,6 +483,11 @@ __device__ __forceinline__ bool fused_tile_of_wg(const FusedArgs<PHYS> &A, int b
     return true;
 }
 
+// zeroes the per-XCD round counters before a launch with rounds in step
+__global__ void k_zero_u32(unsigned *a, int n) {
+    if ((int)threadIdx.x < n) a[threadIdx.x] = 0u;
+}
+
 template <int PHYS, int ZW, int NROW, int EPI>
 __global__ void __launch_bounds__(ZW *NROW) k_fused_step(FusedArgs<PHYS> A) {
     __shared__ FusedLds<fused_nf(PHYS, EPI), ZW, NROW> S;
@@ -651,7 +656,9 @@ static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float
     const bool lockstep = fused_lockstep_of(p, PHYS, EPI, ntiles);
     if (lockstep && p->fused_sync) {
         A.per_xcd = p->cu_count / 8;
-        WFD_TRY(hipMemsetAsync(p->fused_sync, 0, 8 * sizeof(unsigned), p->stream));
+        // a kernel, not a memset, so that the step can be captured in a graph
+        k_zero_u32<<<1, 64, 0, p->stream>>>(p->fused_sync, 8);
+        WFD_LAUNCHED();
     }
     k_fused_step<PHYS, ZW, NROW, EPI><<<dim3((unsigned)ntiles), dim3(ZW * NROW), 0, p->stream>>>(A);
     return true;

@@ -79,11 +79,73 @@ static inline bool time_step(Prop *p, bool linear) {
 // then extract the registered receivers at it -- the loop of test/spacetime.jl:486-492
 // plus extraction, as one call.
 static inline bool time_step_inject_extract(Prop *p, long it, bool linear) {
+    if (!registered_it_ok(p, it, it)) return false;  // before the step: nothing changes on error
     if (!p->begin(p->all_mask())) return false;
     if (!step_device(p, linear)) return false;
     if (!inject_registered(p, it)) return false;
     if (!extract_registered(p, it)) return false;
     return p->end(step_writes(p, linear));
+}
+
+// nsteps calls of time_step_inject_extract (time indices it0 .. it0+nsteps-1) as a HIP graph:
+// two steps (the period of the pCur/pOld swap) are captured once on the handle's stream, with
+// the injection and extraction reading the time index from a device counter that the graph
+// advances, and the graph is launched nsteps/2 times; an odd last step runs as a call.  The
+// kernels, their arguments and their order are those of the calls one by one.  The handle must
+// be in DEVICE mode (the coherent mode copies host arrays inside every call) and use the fused
+// step (the two-pass step allocates its scratch on first use).  Returns when the steps are done.
+static inline bool time_loop_graph(Prop *p, long it0, long nsteps) {
+    if (nsteps <= 0) return true;
+    if (p->coherence != DEVICE) {
+        set_error("TimeStepInjectExtractGraph: the handle must be in DEVICE mode");
+        return false;
+    }
+    if (p->dim != 3 || !p->fused) {
+        set_error("TimeStepInjectExtractGraph: needs the fused 3D step");
+        return false;
+    }
+    if (!registered_it_ok(p, it0, it0 + nsteps - 1)) return false;
+    const long npairs = nsteps / 2;
+    if (npairs > 0) {
+        if (!p->it_dev) WFD_TRY(hipMalloc(&p->it_dev, sizeof(long)));
+        k_set_i64<<<1, 1, 0, p->stream>>>(p->it_dev, it0);
+        WFD_LAUNCHED();
+        WFD_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+        long swaps = 0;  // step_device swaps the pCur/pOld (mCur/mOld) buffers on the host as it records
+        bool ok = true;
+        for (int k = 0; k < 2 && ok; k++) {
+            ok = step_device(p, false);
+            if (!ok) break;
+            swaps++;
+            ok = inject_registered(p, 0, p->it_dev) && extract_registered(p, 0, p->it_dev);
+            if (ok) {
+                k_inc_i64<<<1, 1, 0, p->stream>>>(p->it_dev);
+                ok = hipGetLastError() == hipSuccess;
+            }
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(p->stream, &graph);
+        hipGraphExec_t exec = nullptr;
+        ok = ok && ec == hipSuccess && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        for (long r = 0; r < npairs && ok; r++) ok = hipGraphLaunch(exec, p->stream) == hipSuccess;
+        if (ok) ok = hipStreamSynchronize(p->stream) == hipSuccess;
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!ok) {
+            // the buffers swapped an even number of times after a complete capture; if the
+            // capture broke off, undo the swaps it recorded
+            if (swaps & 1) p->swap_p();
+            if (g_last_error.empty()) set_error("TimeStepInjectExtractGraph: graph capture or launch failed");
+            (void)hipGetLastError();
+            return false;
+        }
+    }
+    if (nsteps & 1) {
+        const long it = it0 + nsteps - 1;
+        if (!step_device(p, false) || !inject_registered(p, it) || !extract_registered(p, it)) return false;
+        WFD_TRY(hipStreamSynchronize(p->stream));
+    }
+    return true;
 }
 
 // the registered injection / extraction alone, between other calls (Born loops: step, Born

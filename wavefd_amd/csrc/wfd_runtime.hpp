@@ -188,6 +188,7 @@ struct Prop {
     unsigned *fused_sync = nullptr;  // per-XCD round counters of the persistent fused launch
     hipEvent_t ev = nullptr;         // orders this handle's stream against a batch's (lazy)
     bool born_field = false;         // Iso: mCur/mOld hold the coupled Born step's second field
+    long *it_dev = nullptr;          // device time index of graph-replayed loops (lazy)
     int dim = 3;                     // 2: a 2D propagator (ny = 1, two-pass 2D kernels)
 
     Mask model_mask() const {
@@ -231,6 +232,7 @@ struct Prop {
     ~Prop() {
         if (stream) hipStreamSynchronize(stream);
         if (fused_sync) (void)hipFree(fused_sync);
+        if (it_dev) (void)hipFree(it_dev);
         if (ev) (void)hipEventDestroy(ev);
         for (int i = 0; i < F_COUNT; i++) {
             if (f[i].d) hipFree(f[i].d);
@@ -456,16 +458,40 @@ static inline bool rminus(Prop *p, const float *i0, const float *i1, const float
 
 // ---- registered injection / extraction ------------------------------------------------
 
-static inline bool inject_registered(Prop *p, long it) {
+// the time indices it (1-based) the registered points accept: the samples of the data, or the
+// fine steps of a fused interpolation
+static inline bool registered_it_ok(const Prop *p, long first, long last) {
+    if (p->src.valid()) {
+        const long n = p->src_nh > 0 ? p->src_nm : p->src_nt;
+        if (first < 1 || last > n) {
+            set_error("time index out of range of the registered sources (1.." + std::to_string(n) + ")");
+            return false;
+        }
+    }
+    if (p->rcv.valid()) {
+        const long n = p->rcv_nh > 0 ? p->rcv_nm : p->rcv_nt;
+        if (first < 1 || last > n) {
+            set_error("time index out of range of the registered receivers (1.." + std::to_string(n) + ")");
+            return false;
+        }
+    }
+    return true;
+}
+
+// itp (optional, device): the kernels read the time index there (graph replays; the caller has
+// checked its range); otherwise it is checked here
+static inline bool inject_registered(Prop *p, long it, const long *itp = nullptr) {
     if (!p->src.valid()) return true;
+    if (!itp && !registered_it_ok(p, it, it)) return false;
     const int nb = (p->src.ngroups + 255) / 256;
     auto inject = [&](float *field) {
         if (p->src_nh > 0)
             k_inject_groups_interp<<<nb, 256, 0, p->stream>>>(field, p->src_data, it, p->src_nt, p->src_h, p->src_nh,
-                                                              p->src_nm, p->src.pts, p->src.gstart, p->src.ngroups);
+                                                              p->src_nm, p->src.pts, p->src.gstart, p->src.ngroups,
+                                                              itp);
         else
             k_inject_groups<<<nb, 256, 0, p->stream>>>(field, p->src_data, it, p->src_nt, p->src.pts, p->src.gstart,
-                                                       p->src.ngroups);
+                                                       p->src.ngroups, itp);
     };
     if (p->src_fields & 1) {
         inject(p->f[F_PCUR].d);
@@ -478,17 +504,18 @@ static inline bool inject_registered(Prop *p, long it) {
     return true;
 }
 
-static inline bool extract_registered(Prop *p, long it) {
+static inline bool extract_registered(Prop *p, long it, const long *itp = nullptr) {
     if (!p->rcv.valid()) return true;
+    if (!itp && !registered_it_ok(p, it, it)) return false;
     const int nb = (p->rcv.ngroups + 255) / 256;
     const int fld = (p->rcv_fields == 2 && (p->phys != ISO || p->born_field)) ? F_MCUR : F_PCUR;
     if (p->rcv_nh > 0)
         k_extract_groups_interp<<<nb, 256, 0, p->stream>>>(p->rcv_data, p->f[fld].d, it, p->rcv_nt, p->rcv_h,
                                                            p->rcv_nh, p->rcv_nm, p->rcv.pts, p->rcv.gstart,
-                                                           p->rcv.ngroups);
+                                                           p->rcv.ngroups, itp);
     else
         k_extract_groups<<<nb, 256, 0, p->stream>>>(p->rcv_data, p->f[fld].d, it, p->rcv_nt, p->rcv.pts,
-                                                    p->rcv.gstart, p->rcv.ngroups);
+                                                    p->rcv.gstart, p->rcv.ngroups, itp);
     WFD_LAUNCHED();
     return true;
 }
