@@ -68,6 +68,9 @@ void Prop2DAcoVTIDenQ_DEO2_FDTD_SetHostCoherence(void *p, long mode);
 void Prop2DAcoVTIDenQ_DEO2_FDTD_SyncToDevice(void *p);
 void Prop2DAcoVTIDenQ_DEO2_FDTD_SyncToHost(void *p);
 float *Prop2DAcoVTIDenQ_DEO2_FDTD_GetDevicePtr(void *p, const char *name);
+/* 1 (default): the fused single-pass step kernel; 0: the two passes (sandwich, then D- and
+ * the update); the results are the same */
+void Prop2DAcoVTIDenQ_DEO2_FDTD_SetFusedStep(void *p, long fused);
 /* the handle's HIP stream (device-mode calls are ordered on it) and a wait for it */
 void *Prop2DAcoVTIDenQ_DEO2_FDTD_GetStream(void *p);
 void Prop2DAcoVTIDenQ_DEO2_FDTD_Synchronize(void *p);

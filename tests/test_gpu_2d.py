@@ -310,3 +310,36 @@ def test_2000_steps(kind):
     assert np.array_equal(_get(prop, "PCur"), orc.pCur)
     if kind != "iso":
         assert np.array_equal(_get(prop, "MCur"), orc.mCur)
+
+
+FUSED_SHAPES = [(37, 45), (100, 300), (70, 520)]  # (nx, nz): one strip and chunk, several of each
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("fs", [False, True])
+@pytest.mark.parametrize("shape", FUSED_SHAPES)
+@pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
+def test_fused_and_two_pass_steps_bit_exact(kind, shape, fs, fused):
+    import wavefd_amd.prop2d as w2
+    prop, orc = _pair(shape, fs, 51) if kind == "iso" else _apair(kind, shape, fs, 51)
+    w2.set_fused_step(prop, fused)
+    for _ in range(9):
+        w2.propagateforward(prop)
+        orc.time_step()
+    names = ("pCur", "pOld", "pSpace") + (("mCur", "mOld", "mSpace") if kind != "iso" else ())
+    for name in names:
+        got = _get(prop, _GET[name]) if kind != "iso" else _get(prop, {"pCur": "PCur", "pOld": "POld",
+                                                                        "pSpace": "PSpace"}[name])
+        assert np.array_equal(got, getattr(orc, name)), name
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_vti_linear_fused_and_two_pass(fused):
+    import wavefd_amd.prop2d as w2
+    prop, orc = _apair("vti", (100, 300), True, 53)
+    w2.set_fused_step(prop, fused)
+    for _ in range(5):
+        w2.propagateforward_linear(prop)
+        orc.time_step_linear()
+    for name in ("pCur", "pOld", "mCur", "mOld", "pSpace"):
+        assert np.array_equal(_get(prop, _GET[name]), getattr(orc, name)), name

@@ -179,6 +179,7 @@ _ISO2D = {
     "GetDevicePtr": (_P, [_P, ctypes.c_char_p]),
     "GetStream": (_P, [_P]),
     "Synchronize": (None, [_P]),
+    "SetFusedStep": (None, [_P, _L]),
     "LastError": (ctypes.c_char_p, []),
     "ClearError": (None, []),
 }
@@ -215,6 +216,7 @@ _ANISO2D = {
     "GetDevicePtr": (_P, [_P, ctypes.c_char_p]),
     "GetStream": (_P, [_P]),
     "Synchronize": (None, [_P]),
+    "SetFusedStep": (None, [_P, _L]),
     "LastError": (ctypes.c_char_p, []),
     "ClearError": (None, []),
 }

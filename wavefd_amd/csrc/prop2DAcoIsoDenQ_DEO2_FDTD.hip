@@ -86,6 +86,7 @@ float *Prop2DAcoIsoDenQ_DEO2_FDTD_GetDevicePtr(void *p, const char *name) {
     const int i = wfd::field_by_name(name);
     return (i >= 0 && P2(p)->f[i].used) ? P2(p)->f[i].d : nullptr;
 }
+void Prop2DAcoIsoDenQ_DEO2_FDTD_SetFusedStep(void *p, long fused) { P2(p)->fused = fused ? 1 : 0; }
 void *Prop2DAcoIsoDenQ_DEO2_FDTD_GetStream(void *p) { return (void *)P2(p)->stream; }
 void Prop2DAcoIsoDenQ_DEO2_FDTD_Synchronize(void *p) {
     if (hipStreamSynchronize(P2(p)->stream) != hipSuccess) wfd::set_error("Synchronize: stream error");

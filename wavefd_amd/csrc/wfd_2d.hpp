@@ -10,6 +10,7 @@
 //   * the mirrored operand is loaded with its sign flipped, so each expression keeps the
 //     reference's terms and order (-(-a) = a exactly).
 #pragma once
+#include "wfd_2d_fused.hpp"
 #include "wfd_kernels.hpp"
 #include "wfd_runtime.hpp"
 
@@ -201,20 +202,6 @@ __global__ void k2_iso_acc(long n, const float *__restrict__ v, const float *__r
 
 // ---- 2D VTI / TTI (src/prop2DAcoVTIDenQ_DEO2_FDTD.h:686-1477,
 // src/prop2DAcoTTIDenQ_DEO2_FDTD.h:667-1717)
-
-// the 2D material sandwich (VTI :785-793, TTI :775-783 on the rotated derivatives); the 2D
-// text multiplies the coupling coefficient left to right, B * F * A * sqrt(1 - A * A)
-__device__ __forceinline__ void aniso2_material(float dPx, float dPz, float dMx, float dMz, float eps, float eta,
-                                                float f, float b, float &tPx, float &tPz, float &tMx, float &tMz) {
-    const float E = 1 + 2 * eps;
-    const float A = eta;
-    const float F = f;
-    const float B = b;
-    tPx = B * E * dPx;
-    tPz = B * (1 - F * A * A) * dPz + B * F * A * sqrtf(1 - A * A) * dMz;
-    tMx = B * (1 - F) * dMx;
-    tMz = B * F * A * sqrtf(1 - A * A) * dPz + B * (1 - F + F * A * A) * dMz;
-}
 
 // VTI (ROT false) and TTI (ROT true) sandwich: t = material(D+ p, D+ m), the TTI derivatives
 // rotated into the symmetry axis' frame first (g1 = cos dx - sin dz, g3 = sin dx + cos dz)
@@ -445,6 +432,18 @@ __global__ void __launch_bounds__(256) k2_dplus_coef(Grid g, float4 cc, const fl
 static inline bool step_2d(Prop *p, bool linear) {
     const Grid &g = p->g;
     Field *f = p->f;
+    if (p->fused) {
+        if (p->phys == TTI && linear) {
+            set_error("2D TTI: there is no linear step");
+            return false;
+        }
+        if (p->nx * p->nz >= (1L << 31)) {
+            set_error("2D fused step: fewer than 2^31 cells");
+            return false;
+        }
+        return p->phys == ISO ? launch_fused2<ISO>(p, linear)
+                              : (p->phys == VTI ? launch_fused2<VTI>(p, linear) : launch_fused2<TTI>(p, linear));
+    }
     if (p->phys == ISO) {
         float *t[2];
         for (int i = 0; i < 2; i++)

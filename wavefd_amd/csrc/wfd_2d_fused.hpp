@@ -1,0 +1,348 @@
+// wfd_2d_fused.hpp -- single-pass fused 2D time step (D+ sandwich, D-, leapfrog with Q) for the
+// 2D Iso, VTI and TTI propagators (src/prop2DAco{Iso,VTI,TTI}DenQ_DEO2_FDTD.h timeStep): one
+// kernel replaces the two passes, so the temporaries never reach HBM.
+//
+// A workgroup of F2W threads owns a strip of F2W consecutive z (one thread per z, lanes
+// coalesced), whose inner F2O = F2W - 7 cells are its outputs (D- reads temporaries at -4..+3),
+// and marches over a chunk of x.  Per thread: p(x..x+7) of its column (the D+x operand) and the
+// x-temporaries of planes x-4..x+3 (the D-x operand) in registers.  At step x the strip of
+// p(x+3) is staged in LDS with a 7-cell border, the sandwich at (x+3, z) is formed (D+z from
+// LDS), its x-temporary pushed into the queue and its z-temporary written into a 4-plane LDS
+// ring; then the output threads apply D- and the update at plane x.  There is no second
+// transverse axis, so the halo is 7 cells of 256 and the step moves the compulsory bytes plus
+// the strips' borders.
+//
+// TTI: the weighted differences D-x(cos g1), D-z(sin g1), D-x(sin g3), D-z(cos g3) read products
+// of a weight and a temporary at neighbouring cells; each product is formed once, when its
+// temporary is (the reference's cosTheta[i] * inPG1[i] at the same i), and queued or stored in
+// the ring instead of the temporary.
+//
+// Semantics match the two-pass kernels (wfd_2d.hpp) exactly: temporaries are 0 outside the
+// update region, the free surface uses the odd mirror for D+ and the even mirror for D-, the x
+// derivative vanishes at kz = 0 and L = 0 there, expression order unchanged.
+#pragma once
+#include "wfd_fused.hpp"
+#include "wfd_kernels.hpp"
+#include "wfd_runtime.hpp"
+
+namespace wfd {
+
+constexpr int F2W = 256;      // threads (lanes along z) per workgroup
+constexpr int F2O = F2W - 7;  // output cells along z per workgroup
+
+template <int PHYS>
+struct Fused2Args {
+    Grid g;
+    float dt2;
+    int cx, nzt, nxc;
+    const float *pCur, *mCur;
+    float *pOld, *mOld, *pSpace, *mSpace;
+    const float *v, *b, *q, *eps, *eta, *f, *sT, *cT;
+};
+
+template <int PHYS>
+struct Fused2Lds {
+    static constexpr int NF = PHYS == ISO ? 1 : 2;  // wavefields
+    static constexpr int NZ = PHYS == TTI ? 2 : 1;  // z-direction operands per field
+    float sp[2][NF][F2W + 7];                       // p (m) of plane x+3, cells z0-7 .. z0+F2W-1
+    float sz[4][NF][NZ][F2W];                       // z operands of planes x .. x+3 (ring)
+};
+
+// the 2D material sandwich (src/prop2DAcoVTIDenQ_DEO2_FDTD.h:785-793,
+// src/prop2DAcoTTIDenQ_DEO2_FDTD.h:775-783 on the rotated derivatives); the 2D
+// text multiplies the coupling coefficient left to right, B * F * A * sqrt(1 - A * A)
+__device__ __forceinline__ void aniso2_material(float dPx, float dPz, float dMx, float dMz, float eps, float eta,
+                                                float f, float b, float &tPx, float &tPz, float &tMx, float &tMz) {
+    const float E = 1 + 2 * eps;
+    const float A = eta;
+    const float F = f;
+    const float B = b;
+    tPx = B * E * dPx;
+    tPz = B * (1 - F * A * A) * dPz + B * F * A * sqrtf(1 - A * A) * dMz;
+    tMx = B * (1 - F) * dMx;
+    tMz = B * F * A * sqrtf(1 - A * A) * dPz + B * (1 - F + F * A * A) * dMz;
+}
+
+__device__ __forceinline__ float sten8v(const float *a) {
+    return C1 * (-a[3] + a[4]) + C2 * (-a[2] + a[5]) + C3 * (-a[1] + a[6]) + C4 * (-a[0] + a[7]);
+}
+
+template <int PHYS, bool FST>
+__device__ __forceinline__ void fused2_body(const Fused2Args<PHYS> &A, Fused2Lds<PHYS> &S, int zt, int xc) {
+    constexpr int NF = Fused2Lds<PHYS>::NF;
+    constexpr int NQ = PHYS == TTI ? 2 : 1;  // x-queues per field
+    constexpr int NM = PHYS == ISO ? 1 : (PHYS == VTI ? 4 : 6);
+    constexpr int NO = 2 + NF;  // v, q, pOld (, mOld); b comes with the sandwich's material
+    const Grid &g = A.g;
+    const int nx = (int)g.nx, nz = (int)g.nz;
+    const int t = threadIdx.x;
+    const int z0 = zt * F2O, z = z0 - 4 + t;
+    const int xs = xc * A.cx;
+    const int xe = xs + A.cx < nx ? xs + A.cx : nx;
+    const int zlo = FST ? 0 : 4;
+    const bool zU = z >= zlo && z < nz - 4;
+    const bool out_thread = t >= 4 && t < 4 + F2O && z < nz;
+    // a column zz of the strip: its offset in a row (clamped into the grid), the odd mirror's
+    // sign above a free surface, zero outside
+    auto col = [&](int zz, uint32_t &sign, bool &ok) -> int {
+        sign = 0;
+        ok = false;
+        if (zz >= 0 && zz < nz) {
+            ok = true;
+            return zz;
+        }
+        if (FST && zz < 0 && -zz < nz) {
+            sign = 0x80000000u;
+            ok = true;
+            return -zz;
+        }
+        return 0;
+    };
+    uint32_t own_sign, b_sign = 0;
+    bool own_ok, b_ok = false;
+    const int own = col(z, own_sign, own_ok);
+    const int moff = zU ? own : 0;
+    const int ooff = out_thread ? own : 0;
+    // border cells of the staged strip: entries 0..2 (z0-7 .. z0-5) and F2W+3 .. F2W+6
+    const bool has_b = t < 7;
+    const int bent = t < 3 ? t : F2W + t;
+    int boff = 0;
+    if (has_b) boff = col(z0 - 7 + bent, b_sign, b_ok);
+    const float *fld[2] = {A.pCur, A.mCur};
+    const float *mat[6] = {A.b, A.eps, A.eta, A.f, A.sT, A.cT};
+    const float *oin[4] = {A.v, A.q, A.pOld, A.mOld};
+    const long nzl = nz;
+    auto ld = [&](const float *a, int xx, int off) -> float {
+        const int xcl = xx < 0 ? 0 : (xx >= nx ? nx - 1 : xx);
+        return a[(long)xcl * nzl + off];
+    };
+    auto fixp = [&](float v, int xx) -> float {
+        v = (own_ok && xx >= 0 && xx < nx) ? v : 0.0f;
+        if constexpr (FST) v = flip(v, own_sign);
+        return v;
+    };
+    auto fixb = [&](float v, int xx) -> float {
+        v = (b_ok && xx >= 0 && xx < nx) ? v : 0.0f;
+        if constexpr (FST) v = flip(v, b_sign);
+        return v;
+    };
+
+    float pq[NF][8], pn[NF], bn[NF], oxq[NF][NQ][8], bq[4];
+    float mc[NM], mnx[NM], oc[NO], onx[NO];
+    const int xfirst = xs - 7;
+#pragma unroll
+    for (int fi = 0; fi < NF; fi++) {
+#pragma unroll
+        for (int k = 0; k < NQ; k++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) oxq[fi][k][j] = 0.0f;
+        pq[fi][0] = 0.0f;
+#pragma unroll
+        for (int j = 1; j < 8; j++) pq[fi][j] = fixp(ld(fld[fi], xfirst + j - 1, own), xfirst + j - 1);
+        pn[fi] = ld(fld[fi], xfirst + 7, own);
+        bn[fi] = ld(fld[fi], xfirst + 3, boff);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) bq[j] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NM; k++) mnx[k] = ld(mat[k], xfirst + 3, moff);
+#pragma unroll
+    for (int k = 0; k < NO; k++) onx[k] = ld(oin[k], xfirst, ooff);
+
+    auto iter = [&](auto U, int x) {
+        constexpr int u = decltype(U)::value;  // x - xfirst = u (mod 4)
+        constexpr int s2 = (u + 3) & 1;        // strip buffer of plane x+3
+        constexpr int sw = (u + 3) & 3;        // ring slot of plane x+3 (written now)
+        constexpr int sr = u & 3;              // ring slot of plane x (read now)
+        // ---- 1. shift the queues, stage plane x+3 ----------------------------------------
+#pragma unroll
+        for (int fi = 0; fi < NF; fi++) {
+#pragma unroll
+            for (int j = 0; j < 7; j++) pq[fi][j] = pq[fi][j + 1];
+            pq[fi][7] = fixp(pn[fi], x + 7);
+            S.sp[s2][fi][t + 3] = pq[fi][3];
+            if (has_b) S.sp[s2][fi][bent] = fixb(bn[fi], x + 3);
+        }
+#pragma unroll
+        for (int k = 0; k < NM; k++) mc[k] = mnx[k];
+#pragma unroll
+        for (int k = 0; k < NO; k++) oc[k] = onx[k];
+        // ---- 2. request the operands of step x+1 ---------------------------------------
+#pragma unroll
+        for (int fi = 0; fi < NF; fi++) {
+            pn[fi] = ld(fld[fi], x + 8, own);
+            bn[fi] = ld(fld[fi], x + 4, boff);
+        }
+#pragma unroll
+        for (int k = 0; k < NM; k++) mnx[k] = ld(mat[k], x + 4, moff);
+#pragma unroll
+        for (int k = 0; k < NO; k++) onx[k] = ld(oin[k], x + 1, ooff);
+        __syncthreads();
+        // ---- 3. sandwich at (x+3, z) ---------------------------------------------------
+        const int xp = x + 3;
+        float tq[NF][NQ], tz[NF][2];
+#pragma unroll
+        for (int fi = 0; fi < NF; fi++) tq[fi][0] = tq[fi][NQ - 1] = tz[fi][0] = tz[fi][1] = 0.0f;
+        if (xp >= 4 && xp < nx - 4 && zU) {
+            float dx[NF], dz[NF];
+#pragma unroll
+            for (int fi = 0; fi < NF; fi++) {
+                float a[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) a[j] = S.sp[s2][fi][t + j];  // cells z-3 .. z+4
+                dz[fi] = g.idz * sten8v(a);
+                dx[fi] = (FST && z == 0) ? 0.0f : g.idx * sten8v(pq[fi]);
+            }
+            if constexpr (PHYS == ISO) {
+                tq[0][0] = mc[0] * dx[0];
+                tz[0][0] = mc[0] * dz[0];
+            } else if constexpr (PHYS == VTI) {
+                aniso2_material(dx[0], dz[0], dx[1], dz[1], mc[1], mc[2], mc[3], mc[0], tq[0][0], tz[0][0], tq[1][0],
+                                tz[1][0]);
+            } else {
+                const float s = mc[4], c = mc[5];
+                const float p1 = c * dx[0] - s * dz[0], p3 = s * dx[0] + c * dz[0];
+                const float m1 = c * dx[1] - s * dz[1], m3 = s * dx[1] + c * dz[1];
+                float t1p, t3p, t1m, t3m;
+                aniso2_material(p1, p3, m1, m3, mc[1], mc[2], mc[3], mc[0], t1p, t3p, t1m, t3m);
+                // operands of the weighted differences: x (queues) cos g1, sin g3; z (ring)
+                // sin g1, cos g3
+                tq[0][0] = c * t1p;
+                tq[0][1] = s * t3p;
+                tq[1][0] = c * t1m;
+                tq[1][1] = s * t3m;
+                tz[0][0] = s * t1p;
+                tz[0][1] = c * t3p;
+                tz[1][0] = s * t1m;
+                tz[1][1] = c * t3m;
+            }
+        }
+#pragma unroll
+        for (int fi = 0; fi < NF; fi++) {
+#pragma unroll
+            for (int k = 0; k < NQ; k++) {
+#pragma unroll
+                for (int j = 0; j < 7; j++) oxq[fi][k][j] = oxq[fi][k][j + 1];
+                oxq[fi][k][7] = tq[fi][k];
+            }
+#pragma unroll
+            for (int k = 0; k < Fused2Lds<PHYS>::NZ; k++) S.sz[sw][fi][k][t] = tz[fi][k];
+        }
+        bq[sw] = mc[0];
+        // ---- 4. D- and leapfrog at plane x -------------------------------------------
+        if (x < xs || x >= xe || !out_thread) return;
+        const bool upd = x >= 4 && x < nx - 4 && zU;
+        const long k0 = (long)x * nzl + ooff;
+        if (!upd) {
+            if (A.pSpace) {
+                A.pSpace[k0] = 0.0f;
+                if constexpr (NF == 2) A.mSpace[k0] = 0.0f;
+            }
+            return;
+        }
+        float L[NF];
+#pragma unroll
+        for (int fi = 0; fi < NF; fi++) {
+            if (FST && z == 0) {
+                L[fi] = 0.0f + 0.0f;
+                continue;
+            }
+            // the z operands at z-4 .. z+3 (even mirror above a free surface: t[-j] = t[j-1])
+            auto zcell = [&](int j) -> int {
+                const int zz = z - 4 + j;
+                const int zm = (FST && zz < 0) ? -zz - 1 : zz;
+                return zm - (z0 - 4);
+            };
+            float a[8];
+            if constexpr (PHYS == TTI) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) a[j] = S.sz[sr][fi][0][zcell(j)];
+                const float B1 = sten8v(a);
+#pragma unroll
+                for (int j = 0; j < 8; j++) a[j] = S.sz[sr][fi][1][zcell(j)];
+                const float B3 = sten8v(a);
+                const float A1 = sten8v(oxq[fi][0]);
+                const float A3 = sten8v(oxq[fi][NQ - 1]);
+                const float o1 = g.idx * A1 - g.idz * B1;
+                const float o3 = g.idx * A3 + g.idz * B3;
+                L[fi] = o1 + o3;
+            } else {
+                const float dxm = g.idx * sten8v(oxq[fi][0]);
+#pragma unroll
+                for (int j = 0; j < 8; j++) a[j] = S.sz[sr][fi][0][zcell(j)];
+                const float dzm = g.idz * sten8v(a);
+                L[fi] = dxm + dzm;
+            }
+        }
+        const float V = oc[0];
+        const float dt2V2_B = A.dt2 * V * V / bq[sr];
+        const float qk = oc[1];
+        A.pOld[k0] = leapfrog(dt2V2_B, L[0], qk, pq[0][0], oc[2]);
+        if (A.pSpace) A.pSpace[k0] = L[0];
+        if constexpr (NF == 2) {
+            A.mOld[k0] = leapfrog(dt2V2_B, L[1], qk, pq[1][0], oc[3]);
+            if (A.mSpace) A.mSpace[k0] = L[1];
+        }
+    };
+
+    // steps x < xs only fill the queues; steps x >= xe (at most 3) store nothing
+    for (int xb = xfirst; xb < xe; xb += 4) {
+        iter(IC<0>{}, xb);
+        iter(IC<1>{}, xb + 1);
+        iter(IC<2>{}, xb + 2);
+        iter(IC<3>{}, xb + 3);
+    }
+}
+
+template <int PHYS>
+__global__ void __launch_bounds__(F2W) k2_fused_step(Fused2Args<PHYS> A) {
+    __shared__ Fused2Lds<PHYS> S;
+    const int bid = (int)blockIdx.x;
+    const int zt = bid % A.nzt, xc = bid / A.nzt;
+    if (xc >= A.nxc) return;
+    if (A.g.fs && zt == 0)
+        fused2_body<PHYS, true>(A, S, zt, xc);
+    else
+        fused2_body<PHYS, false>(A, S, zt, xc);
+}
+
+// x-planes per chunk: enough workgroups for about four per CU, chunks of at least 32 planes
+// (each pays a 7-step warm-up)
+static inline int fused2_cx(const Prop *p, int nzt) {
+    const long want = 4L * (p->cu_count > 0 ? p->cu_count : 256);
+    long nxc = (want + nzt - 1) / nzt;
+    nxc = std::max(1L, std::min(nxc, p->nx / 32 > 0 ? p->nx / 32 : 1L));
+    return (int)((p->nx + nxc - 1) / nxc);
+}
+
+template <int PHYS>
+static inline bool launch_fused2(Prop *p, bool linear) {
+    const Field *f = p->f;
+    Fused2Args<PHYS> A;
+    A.g = p->g;
+    A.dt2 = p->dt * p->dt;
+    A.nzt = (int)((p->nz + F2O - 1) / F2O);
+    A.cx = fused2_cx(p, A.nzt);
+    A.nxc = (int)((p->nx + A.cx - 1) / A.cx);
+    A.pCur = f[F_PCUR].d;
+    A.pOld = f[F_POLD].d;
+    A.pSpace = linear ? nullptr : f[F_PSPACE].d;
+    A.mCur = PHYS == ISO ? nullptr : f[F_MCUR].d;
+    A.mOld = PHYS == ISO ? nullptr : f[F_MOLD].d;
+    A.mSpace = (PHYS == ISO || linear) ? nullptr : f[F_MSPACE].d;
+    A.v = f[F_V].d;
+    A.b = f[F_B].d;
+    A.q = f[F_Q].d;
+    A.eps = PHYS == ISO ? nullptr : f[F_EPS].d;
+    A.eta = PHYS == ISO ? nullptr : f[F_ETA].d;
+    A.f = PHYS == ISO ? nullptr : f[F_F].d;
+    A.sT = PHYS == TTI ? f[F_ST].d : nullptr;
+    A.cT = PHYS == TTI ? f[F_CT].d : nullptr;
+    if (PHYS == ISO) A.mCur = A.pCur, A.mOld = A.pOld;  // never read: one wavefield
+    const long nwg = (long)A.nzt * A.nxc;
+    k2_fused_step<PHYS><<<dim3((unsigned)nwg), dim3(F2W), 0, p->stream>>>(A);
+    WFD_LAUNCHED();
+    return true;
+}
+
+}  // namespace wfd

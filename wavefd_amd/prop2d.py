@@ -301,3 +301,8 @@ def device_ptr(prop, name):
     if not addr:
         raise KeyError(name)
     return addr
+
+
+def set_fused_step(prop, fused=True):
+    """Choose the fused single-pass 2D step (default) or the two passes; the results are the same."""
+    prop._lib.SetFusedStep(prop.p, 1 if fused else 0)
