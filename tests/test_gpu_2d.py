@@ -343,3 +343,30 @@ def test_vti_linear_fused_and_two_pass(fused):
         orc.time_step_linear()
     for name in ("pCur", "pOld", "mCur", "mOld", "pSpace"):
         assert np.array_equal(_get(prop, _GET[name]), getattr(orc, name)), name
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_apply_first_derivatives_host_caller_coefficients(fs):
+    # arbitrary c1..c4 and spacings: the entry point must use the caller's, each in its place
+    # (src/propagatorStaticFunctions.h:10-161 term order, float32, free-surface odd mirror)
+    import wavefd_amd.prop2d as w2
+    nx, nz = 23, 30
+    rng = np.random.default_rng(61)
+    a, c = rng.standard_normal((nx, nz)).astype(F32), rng.standard_normal((nx, nz)).astype(F32)
+    cc = [F32(v) for v in (1.3, -0.21, 0.037, -0.0042)]
+    idx, idz = F32(0.7), F32(1.9)
+    ox, oz = np.full((nx, nz), 9.0, F32), np.full((nx, nz), 9.0, F32)
+    w2.applyFirstDerivatives2D_PlusHalf(fs, nx, nz, cc, idx, idz, a, c, ox, oz)
+
+    def sten(f, i, s):  # f[i+s*j] along a stride, j = -3..4
+        return (cc[0] * (-f(i) + f(i + s)) + cc[1] * (-f(i - s) + f(i + 2 * s)) + cc[2] * (-f(i - 2 * s) + f(i + 3 * s))
+                + cc[3] * (-f(i - 3 * s) + f(i + 4 * s)))
+    wx, wz = np.zeros((nx, nz), F32), np.zeros((nx, nz), F32)
+    for kx in range(4, nx - 4):
+        for kz in range(0 if fs else 4, nz - 4):
+            sx = sten(lambda i: a[i, kz], kx, 1)
+            sz = sten(lambda i: c[kx, i] if i >= 0 else -c[kx, -i], kz, 1)
+            wx[kx, kz] = F32(0) if (fs and kz == 0) else idx * sx
+            wz[kx, kz] = idz * sz
+    assert np.array_equal(ox, wx)
+    assert np.array_equal(oz, wz)
