@@ -162,3 +162,32 @@ def test_offgrid_vs_ongrid_accuracy_3d(F, fs):
     pon = modeling(True)[::2, ::2, ::2]
     r = np.max((poff - pon) / np.abs(pon).max())
     assert r < 0.1
+
+
+def test_device_inject_heavy_collisions_and_empty():
+    # 600 taps on 16 cells (every cell shared by ~37 taps, many blockings): the per-cell sums
+    # must run in the host path's order; an empty point list changes nothing
+    import wavefd_amd as w
+    from tests.devmem import DeviceArray
+    rng = np.random.default_rng(7)
+    n, nr, nt = 4096, 5, 12
+    pts = np.zeros(600, w.SourcePoint32)
+    cells = rng.choice(np.arange(1, n + 1), 16, replace=False)
+    pts["iu"] = rng.choice(cells, 600)
+    pts["ir"] = rng.integers(1, nr + 1, 600)
+    pts["c"] = rng.standard_normal(600).astype(F32)
+    field = rng.standard_normal(n).astype(F32)
+    data = rng.standard_normal((nr, nt)).astype(F32)
+    for nthreads in (1, 3, 8):
+        for it in (1, 7, nt):
+            a = field.copy()
+            w.injectdata(a, w.source_blocking(pts, nthreads), data, it)
+            dev = DeviceArray(field)
+            w.injectdata(dev, w.source_blocking(pts, nthreads), data, it)
+            assert np.array_equal(dev.numpy(), a), (nthreads, it)
+    dev = DeviceArray(field)
+    w.injectdata(dev, [], data, 1)
+    assert np.array_equal(dev.numpy(), field)
+    d2 = DeviceArray(data)
+    w.extractdata(d2, DeviceArray(field), 1, [])
+    assert np.array_equal(d2.numpy(), data)
