@@ -83,6 +83,9 @@ class _ExactStore:
     def slot(self, kind, n):
         return self.buf.ptr + 4 * self.ncell * ((n - 1) * self.nf + kind)
 
+    def begin(self, n):  # forward step n is about to run
+        pass
+
     def out(self, kind, n):  # where the step writes DP_n
         return self.slot(kind, n)
 
@@ -114,6 +117,9 @@ class _CodecStore:
     def code(self, kind, n):
         return self.codes.ptr + 4 * self.words * ((n - 1) * self.nf + kind)
 
+    def begin(self, n):
+        pass
+
     def out(self, kind, n):
         return self.scratch[kind].ptr
 
@@ -136,6 +142,116 @@ class _CodecStore:
             b.free()
 
 
+def checkpoint_fields(nt, nf, k):
+    """Fields of one window a checkpointed store holds for nt steps, checkpoints every k steps:
+    2 nf per checkpoint (ceil(nt / k) of them), k nf for the segment at hand, and the second
+    propagator's state fields (Iso: v b Q pCur pOld pSpace; VTI: 12).  Exact storage: nt nf."""
+    nseg = (nt + k - 1) // k
+    return 2 * nf * nseg + min(k, nt) * nf + (6 if nf == 1 else 12)
+
+
+class _CheckpointStore:
+    """DP_n (DM_n) recomputed instead of stored.  The background wavefields (pCur, pOld[, mCur,
+    mOld]) are copied every k steps of the forward pass (checkpoints: the state before steps 1,
+    k+1, 2k+1, ...).  When a pass asks for DP_n of a segment that is not at hand, the segment's
+    steps run again from its checkpoint on a second propagator holding the same model, Q and
+    source, with the forward pass's own step and injection calls; the kernels are
+    deterministic, so every recomputed DP_n is bit-identical to the one the forward pass formed
+    and the image equals that of exact storage bit for bit.  The forward pass writes the last
+    segment's DP_n straight into the segment buffer, so the adjoint's first segment needs no
+    recomputation.  Costs: one more background pass for the adjoint (and one for a separate Born
+    forward pass, i.e. VTI); memory: ``checkpoint_fields`` instead of nt nf fields.  The second
+    propagator runs on its own stream; the main stream is drained before a segment is
+    overwritten and the second stream before its DP_n are read."""
+
+    def __init__(self, run, k, fused):
+        if int(k) < 1:
+            raise ValueError("checkpoint interval must be >= 1")
+        self.run, self.k, self.fused = run, int(k), fused
+        self.nf, self.nt, self.ncell = run.nf, run.nt, run.ncell
+        self.nseg = (self.nt + self.k - 1) // self.k
+        self.seglen = min(self.k, self.nt)
+        self.ckpt = ginsu.DeviceBuffer(max(4, 4 * self.ncell * 2 * self.nf * self.nseg))
+        self.seg = ginsu.DeviceBuffer(max(4, 4 * self.ncell * self.seglen * self.nf))
+        self.scratch = ginsu.DeviceBuffer(4 * self.ncell * self.nf)
+        self.cached = None  # the segment whose DP_n the segment buffer holds
+        self.prop2 = None
+
+    def _names(self):
+        return ("pCur", "pOld") + (("mCur", "mOld") if self.nf == 2 else ())
+
+    def _ckpt(self, j, i):
+        return self.ckpt.ptr + 4 * self.ncell * (j * 2 * self.nf + i)
+
+    def _slot(self, kind, n):
+        j = (n - 1) // self.k
+        return self.seg.ptr + 4 * self.ncell * ((n - 1 - j * self.k) * self.nf + kind)
+
+    def begin(self, n):
+        if (n - 1) % self.k == 0:
+            r = self.run
+            for i, name in enumerate(self._names()):
+                ginsu.copy(self._ckpt((n - 1) // self.k, i), prop3d.device_ptr(r.prop, name), 4 * self.ncell,
+                           r.stream)
+
+    def out(self, kind, n):
+        j = (n - 1) // self.k
+        if j == self.nseg - 1:
+            self.cached = j
+            return self._slot(kind, n)
+        return self.scratch.ptr + 4 * self.ncell * kind
+
+    def commit(self, kind, n, stream):
+        pass
+
+    def _second(self):
+        r = self.run
+        p2 = r.new_prop()
+        for name in MODEL_FIELDS[r.physics] + ("dtOmegaInvQ",):
+            ginsu.copy(prop3d.device_ptr(p2, name), prop3d.device_ptr(r.prop, name), 4 * self.ncell,
+                       p2._lib.GetStream(p2.p))
+        prop3d.set_sources(p2, [r.src_pts], r.shot.wavelet, fields=1 if self.nf == 1 else 3)
+        prop3d.set_receivers(p2, [], self.nt)
+        return p2
+
+    def _recompute(self, j):
+        r = self.run
+        prop3d.synchronize(r.prop)  # the checkpoints are written and the old segment is read
+        if self.prop2 is None:
+            self.prop2 = self._second()
+        p2 = self.prop2
+        lib, h, stream = p2._lib, p2.p, p2._lib.GetStream(p2.p)
+        for i, name in enumerate(self._names()):
+            ginsu.copy(prop3d.device_ptr(p2, name), self._ckpt(j, i), 4 * self.ncell, stream)
+        for it in range(j * self.k + 1, min((j + 1) * self.k, self.nt) + 1):
+            slots = [self._slot(kind, it) for kind in range(self.nf)]
+            if self.fused:
+                lib.TimeStepSaveDP(h, *slots)
+                lib.InjectRegistered(h, it)
+            else:
+                lib.TimeStepInjectExtract(h, it)
+                lib.ScaleSpatialDerivatives(h)
+                ginsu.copy(slots[0], prop3d.device_ptr(p2, "pSpace"), 4 * self.ncell, stream)
+                if self.nf == 2:
+                    ginsu.copy(slots[1], prop3d.device_ptr(p2, "mSpace"), 4 * self.ncell, stream)
+        p2._check()
+        prop3d.synchronize(p2)
+        self.cached = j
+
+    def get(self, kind, n, stream):
+        j = (n - 1) // self.k
+        if self.cached != j:
+            self._recompute(j)
+        return self._slot(kind, n)
+
+    def free(self):
+        for b in (self.ckpt, self.seg, self.scratch):
+            b.free()
+        if self.prop2 is not None:
+            prop3d.free(self.prop2)
+            self.prop2 = None
+
+
 class BornShot:
     """One shot: source (z, y, x), receiver coordinate arrays, a wavelet of nt samples at dt, all
     in the big grid's coordinates (origin 0)."""
@@ -155,7 +271,9 @@ class _ShotRun:
     model, dv, the window image, the DP (DM) snapshot area and the shot's taps."""
 
     def __init__(self, physics, window, spacing, dt, shot, model, dmodel, nsponge, freesurface, freqQ, qMin,
-                 qInterior, snapshots, codec=None, learned=True):
+                 qInterior, snapshots, codec=None, learned=True, checkpoint=None, fused=True):
+        if checkpoint is not None and (codec is not None or int(checkpoint) < 1):
+            raise ValueError("checkpoint: an interval >= 1, and not together with a codec store")
         self.physics, self.window, self.shot = physics, window, shot
         self.nf = nf = 1 if physics == "iso" else 2
         nz, ny, nx = window.size
@@ -163,17 +281,14 @@ class _ShotRun:
         self.nt = nt = shot.wavelet.shape[1]
         self.ncell = ncell = window.ncells
         self.nsponge, self.freesurface, self.dt = nsponge, freesurface, dt
-        self.prop = prop = _CLS[physics](nz=nz, ny=ny, nx=nx, nsponge=nsponge, dz=dz, dy=dy, dx=dx, dt=dt,
-                                         freesurface=freesurface, freqQ=freqQ, qMin=qMin, qInterior=qInterior)
-        prop3d.coherence(prop, "device")
+        self.ctor = dict(nz=nz, ny=ny, nx=nx, nsponge=nsponge, dz=dz, dy=dy, dx=dx, dt=dt, freesurface=freesurface,
+                         freqQ=freqQ, qMin=qMin, qInterior=qInterior)
+        self.prop = prop = self.new_prop()
         self.lib = prop._lib
         self.stream = stream = prop._lib.GetStream(prop.p)
         for name in MODEL_FIELDS[physics]:
             ginsu.extract(model[name], window, prop3d.device_ptr(prop, name), stream)
-        if codec is None:
-            self.store = _ExactStore(ncell, nt, nf, snapshots)
-        else:
-            self.store = _CodecStore(window.size, nt, nf, codec, learned)
+        self.store = None
         self.dv = ginsu.DeviceBuffer(4 * ncell)
         ginsu.extract(dmodel, window, self.dv.ptr, stream)
         self.gimg = ginsu.DeviceBuffer(4 * ncell)
@@ -187,6 +302,20 @@ class _ShotRun:
         self.rcv_pts = spacetime.linearcoeffs(F32(dz), F32(dy), F32(dx), 0.0, 0.0, 0.0, nz, ny, nx, rz, ry, rx,
                                               fs_index=fsi)
         self.born_data = None
+        if checkpoint is not None:
+            self.store = _CheckpointStore(self, checkpoint, fused)
+        elif codec is None:
+            self.store = _ExactStore(ncell, nt, nf, snapshots)
+        else:
+            self.store = _CodecStore(window.size, nt, nf, codec, learned)
+
+    def new_prop(self):
+        prop = _CLS[self.physics](**self.ctor)
+        prop3d.coherence(prop, "device")
+        return prop
+
+    def begin(self, n):  # forward step n of the background is about to run
+        self.store.begin(n)
 
     def snap(self, kind, n):  # where the step writes DP_n / DM_n (n 1-based)
         return self.store.out(kind, n)
@@ -241,7 +370,8 @@ class _ShotRun:
     def free(self):
         self.dv.free()
         self.gimg.free()
-        self.store.free()
+        if self.store is not None:
+            self.store.free()
         prop3d.free(self.prop)
 
 
@@ -269,6 +399,7 @@ def _run_coupled(runs):
     for it in range(1, nt + 1):
         for r in runs:
             lib, p = r.lib, r.prop.p
+            r.begin(it)
             lib.TimeStepBornCoupled_V(p, r.dv.ptr, r.snap(0, it))
             r.commit(it)
             lib.InjectRegistered(p, it)
@@ -292,6 +423,8 @@ def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
         r.setup_forward()
     for it in range(1, (0 if coupled else nt) + 1):
         if batched:
+            for r in runs:
+                r.begin(it)
             _batch_step(runs, 1, list(zip(*[r.dps(it) for r in runs])))
             for r in runs:
                 r.commit(it)
@@ -299,6 +432,7 @@ def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
             continue
         for r in runs:
             lib, p = r.lib, r.prop.p
+            r.begin(it)
             if fused:
                 lib.TimeStepSaveDP(p, *r.dps(it))
                 r.commit(it)
@@ -360,7 +494,7 @@ def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
 
 def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=20, freesurface=True, freqQ=5.0,
               qMin=0.1, qInterior=100.0, snapshots=None, adjoint_data=None, keep_born_data=True, fused=True,
-              coupled=None, snapshot_codec=None, learned=True):
+              coupled=None, snapshot_codec=None, learned=True, checkpoint=None):
     """Run the three passes of one shot on `window` and add its image into `image`.
 
     model: {'v','b'[,'eps','eta','f']} -> device address of the big grid's field; dmodel: device
@@ -372,15 +506,17 @@ def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=
     results as the separate calls).  coupled (default: Iso with fused): passes 1 and 2 as one
     pass over both wavefields (TimeStepBornCoupled_V; the same results again).  snapshot_codec: a
     ``lcodec.LearnedCodec`` to keep DP (DM) as learned-codec codes instead of exactly (decoded
-    by its network, or by the plain dequantisation with learned=False).  Returns a dict with the
-    Born data and timings."""
+    by its network, or by the plain dequantisation with learned=False).  checkpoint: an interval k
+    to recompute DP (DM) from checkpoints of the background every k steps instead of storing them
+    (``_CheckpointStore``: the same image bit for bit, ``checkpoint_fields`` of memory instead of
+    nt per wavefield).  Returns a dict with the Born data and timings."""
     if physics not in _CLS:
         raise ValueError("physics must be 'iso' or 'vti'")
     if coupled is None:
         coupled = fused and physics == "iso"
     t0 = time.perf_counter()
     run = _ShotRun(physics, window, spacing, dt, shot, model, dmodel, nsponge, freesurface, freqQ, qMin, qInterior,
-                   snapshots, snapshot_codec, learned)
+                   snapshots, snapshot_codec, learned, checkpoint, fused)
     t1 = time.perf_counter()
     try:
         _run_passes([run], fused, None if adjoint_data is None else [adjoint_data], keep_born_data, coupled)
@@ -394,14 +530,15 @@ def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=
 
 def born_shots(physics, windows, spacing, dt, shots, model, dmodel, image, nsponge=20, freesurface=True, freqQ=5.0,
                qMin=0.1, qInterior=100.0, batch=4, snapshot_bytes=None, adjoint_data=None, keep_born_data=True,
-               fused=True):
+               fused=True, checkpoint=None):
     """Born modelling and imaging of many shots (one window each, equal nt), several at a time:
     the shots of a batch run their passes together, each time step of all of them one launch
     of the fused kernel (TimeStepBatch).  Batches are formed in order, at most `batch` shots and,
     if snapshot_bytes is given, at most that many bytes of DP (DM) snapshots (a shot larger than
     that runs alone).  Every shot's results equal born_shot's for it; the images are added into
-    `image` one shot after the other, in shot order.  Returns the list of per-shot Born data (None
-    entries unless keep_born_data)."""
+    `image` one shot after the other, in shot order.  checkpoint: as in born_shot (the bytes a
+    shot counts against snapshot_bytes are then its ``checkpoint_fields``).  Returns the list of
+    per-shot Born data (None entries unless keep_born_data)."""
     if physics not in _CLS:
         raise ValueError("physics must be 'iso' or 'vti'")
     if len(windows) != len(shots):
@@ -414,7 +551,7 @@ def born_shots(physics, windows, spacing, dt, shots, model, dmodel, image, nspon
     batch = max(1, min(int(batch), 8))
     groups, cur, cur_bytes = [], [], 0
     for i, w in enumerate(windows):
-        b = 4 * w.ncells * nt * nf
+        b = 4 * w.ncells * (nt * nf if checkpoint is None else checkpoint_fields(nt, nf, checkpoint))
         if cur and (len(cur) == batch or (snapshot_bytes is not None and cur_bytes + b > snapshot_bytes)):
             groups.append(cur)
             cur, cur_bytes = [], 0
@@ -428,7 +565,7 @@ def born_shots(physics, windows, spacing, dt, shots, model, dmodel, image, nspon
         try:
             for i in g:
                 runs.append(_ShotRun(physics, windows[i], spacing, dt, shots[i], model, dmodel, nsponge, freesurface,
-                                     freqQ, qMin, qInterior, None))
+                                     freqQ, qMin, qInterior, None, checkpoint=checkpoint, fused=fused))
             _run_passes(runs, fused, None if adjoint_data is None else [adjoint_data[i] for i in g], keep_born_data,
                         coupled=fused and physics == "iso" and len(runs) == 1)
             for i, r in zip(g, runs):
