@@ -1,7 +1,8 @@
 """Host logic of the Born driver's checkpointed snapshot store (no GPU): the segment slots of
 every step and wavefield are distinct and inside the segment buffer, checkpoints are taken
-before steps 1, k+1, 2k+1, ..., the forward pass writes the last segment straight into the
-segment buffer, and a pass over the steps in either order recomputes each other segment once."""
+before steps 1, k+1, 2k+1, ..., the coupled forward pass writes the last segment straight into
+the segment buffer, a separate Born pass runs the background segment by segment with no
+recomputation, and the adjoint recomputes every segment but the last one once."""
 import types
 
 import numpy as np
@@ -32,7 +33,7 @@ def store_factory(monkeypatch):
         run = types.SimpleNamespace(nf=nf, nt=nt, ncell=ncell, prop=None, stream=None)
         st = born._CheckpointStore(run, k, True)
         recomputed = []
-        st._recompute = lambda j: (recomputed.append(j), setattr(st, "cached", j))
+        st._recompute = lambda j, first=False: (recomputed.append((j, first)), setattr(st, "cached", j))
         return st, copies, recomputed
 
     return make
@@ -40,8 +41,8 @@ def store_factory(monkeypatch):
 
 @pytest.mark.parametrize("nt,k", [(45, 1), (45, 7), (45, 16), (45, 45), (45, 50), (200, 20)])
 @pytest.mark.parametrize("nf", [1, 2])
-@pytest.mark.parametrize("separate", [False, True])
-def test_slots_checkpoints_and_recompute_schedule(store_factory, nt, k, nf, separate):
+@pytest.mark.parametrize("segmented", [False, True])
+def test_slots_checkpoints_and_recompute_schedule(store_factory, nt, k, nf, segmented):
     st, copies, recomputed = store_factory(nt, nf, k)
     nseg = (nt + k - 1) // k
     slots = {st._slot(kind, n) for n in range(1, nt + 1) for kind in range(nf)
@@ -49,28 +50,32 @@ def test_slots_checkpoints_and_recompute_schedule(store_factory, nt, k, nf, sepa
     seg = [st._slot(kind, n) for n in range(1, min(k, nt) + 1) for kind in range(nf)]
     assert len(set(seg)) == len(seg)
     assert min(seg) == st.seg.ptr and max(seg) + 4 * st.ncell == st.seg.ptr + st.seg.nbytes
-    # forward: checkpoints before steps 1, k+1, ...; the last segment goes to the segment buffer
-    for n in range(1, nt + 1):
-        st.begin(n)
-        for kind in range(nf):
-            o = st.out(kind, n)
-            assert (o in slots) == ((n - 1) // k == nseg - 1)
-    assert len(copies) == 2 * nf * nseg
-    ck = sorted({c[0] for c in copies})
-    assert len(ck) == 2 * nf * nseg and ck[-1] + 4 * st.ncell <= st.ckpt.ptr + st.ckpt.nbytes
-    # a separate Born pass (VTI) reads in forward order: every segment but the last one at hand
-    # is recomputed, and the last one again once an earlier one has replaced it
-    if separate:
+    if not segmented:
+        # coupled forward: checkpoints before steps 1, k+1, ...; the last segment goes to the
+        # segment buffer
         for n in range(1, nt + 1):
+            st.begin(n)
             for kind in range(nf):
-                st.get(kind, n, None)
-        assert recomputed == (list(range(nseg)) if nseg > 1 else [])
+                o = st.out(kind, n)
+                assert (o in slots) == ((n - 1) // k == nseg - 1)
+        assert len(copies) == 2 * nf * nseg
+        ck = sorted({c[0] for c in copies})
+        assert len(ck) == 2 * nf * nseg and ck[-1] + 4 * st.ncell <= st.ckpt.ptr + st.ckpt.nbytes
+        assert recomputed == []
+    else:
+        # separate Born pass: the background of each segment, then its Born steps read it
+        for n in range(1, nt + 1):
+            if (n - 1) % k == 0:
+                st.background_segment((n - 1) // k)
+            for kind in range(nf):
+                assert st.get(kind, n, None) in seg
+        assert recomputed == [(j, True) for j in range(nseg)]
         recomputed.clear()
     # adjoint order: the last segment is at hand, every other one recomputed once
     for n in range(nt, 0, -1):
         for kind in range(nf):
             st.get(kind, n, None)
-    assert recomputed == list(range(nseg - 2, -1, -1))
+    assert recomputed == [(j, False) for j in range(nseg - 2, -1, -1)]
 
 
 def test_memory_formula():

@@ -214,7 +214,10 @@ class _CheckpointStore:
         prop3d.set_receivers(p2, [], self.nt)
         return p2
 
-    def _recompute(self, j):
+    def _recompute(self, j, first=False):
+        """the steps of segment j on the second propagator, their DP_n into the segment buffer:
+        from checkpoint j, or (first) from the second propagator's own state, which is then
+        saved as checkpoint j -- the background pass itself, segment by segment"""
         r = self.run
         prop3d.synchronize(r.prop)  # the checkpoints are written and the old segment is read
         if self.prop2 is None:
@@ -222,7 +225,10 @@ class _CheckpointStore:
         p2 = self.prop2
         lib, h, stream = p2._lib, p2.p, p2._lib.GetStream(p2.p)
         for i, name in enumerate(self._names()):
-            ginsu.copy(prop3d.device_ptr(p2, name), self._ckpt(j, i), 4 * self.ncell, stream)
+            if first:
+                ginsu.copy(self._ckpt(j, i), prop3d.device_ptr(p2, name), 4 * self.ncell, stream)
+            else:
+                ginsu.copy(prop3d.device_ptr(p2, name), self._ckpt(j, i), 4 * self.ncell, stream)
         for it in range(j * self.k + 1, min((j + 1) * self.k, self.nt) + 1):
             slots = [self._slot(kind, it) for kind in range(self.nf)]
             if self.fused:
@@ -243,6 +249,11 @@ class _CheckpointStore:
         if self.cached != j:
             self._recompute(j)
         return self._slot(kind, n)
+
+    def background_segment(self, j):
+        """pass 1 for segment j on the second propagator (which starts from zero fields, like the
+        main one), checkpointing it; a separate Born pass then reads the segment buffer"""
+        self._recompute(j, first=True)
 
     def free(self):
         for b in (self.ckpt, self.seg, self.scratch):
@@ -418,10 +429,13 @@ def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
         raise ValueError("the coupled Born pass is for Iso shots run one at a time with fused steps")
     if coupled:
         _run_coupled(runs)
+    # with checkpointed stores, passes 1 and 2 alternate per segment: the background of a segment
+    # runs on the store's second propagator, then the Born steps of the segment read its DP_n
+    segmented = not coupled and all(isinstance(r.store, _CheckpointStore) for r in runs)
     # 1. background forward, saving DP (and DM)
-    for r in ([] if coupled else runs):
+    for r in ([] if coupled or segmented else runs):
         r.setup_forward()
-    for it in range(1, (0 if coupled else nt) + 1):
+    for it in range(1, (0 if coupled or segmented else nt) + 1):
         if batched:
             for r in runs:
                 r.begin(it)
@@ -450,6 +464,9 @@ def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
     for r in ([] if coupled else runs):
         r.setup_born()
     for it in range(1, (0 if coupled else nt) + 1):
+        if segmented and (it - 1) % runs[0].store.k == 0:
+            for r in runs:
+                r.store.background_segment((it - 1) // r.store.k)
         if batched:
             _batch_step(runs, 2, [[r.dv.ptr for r in runs]] + list(zip(*[r.dps_in(it) for r in runs])))
         for r in runs:
