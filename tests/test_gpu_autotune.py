@@ -55,7 +55,7 @@ def test_results_do_not_depend_on_the_configuration(kind):
 
 @pytest.mark.parametrize("kind", ["vti", "tti"])
 def test_rounds_in_step_change_no_result(kind):
-    # enough workgroups for 16 rounds of one per CU (4096 on 256 CUs): 45 tiles per plane
+    # enough workgroups for 20 rounds of one per CU (5120 on 256 CUs): 45 tiles per plane
     # (128 x 128, 64 x 16 tile) times 128 chunks of 2 x-planes
     shape = (256, 128, 128)
     eff_off, ref = _run(kind, shape, ("64x16", False, 2), nsteps=2)

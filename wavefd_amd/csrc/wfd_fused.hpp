@@ -632,7 +632,7 @@ static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float 
 // rounds in step: the automatic rule, overridden by the handle's setting, else by the
 // environment; never where the rounds cannot be formed
 static inline bool fused_lockstep_of(const Prop *p, int phys, int epi, long ntiles) {
-    const bool can = p->cu_count % 8 == 0 && ntiles >= 16L * p->cu_count;
+    const bool can = p->cu_count % 8 == 0 && ntiles >= 20L * p->cu_count;
     bool lockstep = can && phys != ISO && epi == EPI_NONE;
     if (p->fused_lockstep >= 0)
         lockstep = can && p->fused_lockstep != 0;
@@ -651,8 +651,8 @@ static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float
     // (VTI, TTI: 4 and 8 material fields; measured -20 % fetched bytes, -3 % time for TTI
     // 800^3); for Iso, and for the Born epilogues (VTI Born passes on 200^3..600^3 windows:
     // 6876 against 7032 shots per hour), the waiting costs more than the shared lines save.  It
-    // needs many rounds: 800^3 (21 rounds) TTI 13.31 -> 12.73 ms, VTI 10.36 -> 10.14 ms; 512^3
-    // (9 rounds) TTI 3.43 -> 3.70 ms, VTI 2.79 -> 2.99 ms.
+    // needs many rounds: 800^3 (21 rounds of 256 workgroups) TTI 13.31 -> 12.73 ms, VTI 10.36 ->
+    // 10.14 ms; 512^3 (18 rounds) TTI 3.43 -> 3.70 ms, VTI 2.80 -> 2.98 ms.
     const bool lockstep = fused_lockstep_of(p, PHYS, EPI, ntiles);
     if (lockstep && p->fused_sync) {
         A.per_xcd = p->cu_count / 8;
