@@ -297,3 +297,28 @@ def test_ricker_closed_form():
     # zeros at t = +-1/(sqrt(2) pi f)
     t0 = 1 / (np.sqrt(2) * np.pi * 15.0)
     assert abs(float(w.ricker(15.0, np.array([t0]))[0])) < 1e-6
+
+
+@pytest.mark.parametrize("F", ["hickscoeffs", "linearcoeffs"])
+def test_many_points_equal_one_point_calls(F):
+    # the builders handle all points at once; each point's taps must be those of a call with
+    # that point alone (same order, same bits), with its trace number
+    import wavefd_amd as w
+    fn = getattr(w, F)
+    rng = np.random.default_rng(5)
+    d = (np.float32(10), np.float32(12.5), np.float32(8))
+    n = (30, 25, 40)
+    m = 60
+    z = np.concatenate([rng.uniform(-15, 310, m // 2), d[0] * rng.integers(0, 5, m // 2)])  # near the surface
+    y = rng.uniform(-30, 330, m)
+    x = rng.uniform(-20, 330, m)
+    for fs in (0, 1, 2):
+        allp = fn(*d, 0.0, 0.0, 0.0, *n, z, y, x, fs_index=fs)
+        one = []
+        for i in range(m):
+            p = fn(*d, 0.0, 0.0, 0.0, *n, z[i:i + 1], y[i:i + 1], x[i:i + 1], fs_index=fs)
+            p["ir"] = i + 1
+            one.append(p)
+        one = np.concatenate(one)
+        assert np.array_equal(allp["iu"], one["iu"]) and np.array_equal(allp["ir"], one["ir"])
+        assert np.array_equal(allp["c"].view(np.uint32), one["c"].view(np.uint32))

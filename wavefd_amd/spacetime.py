@@ -54,86 +54,89 @@ def sinc32(x):
 
 
 def _hicks_1d(b, r, alpha):
+    """the 2r taps at offsets -r+1..r of one offset alpha (shape (2r,)) or of an array of them
+    (shape alpha.shape + (2r,))"""
     a = np.arange(1, 2 * r + 1) - r  # i - r for i = 1..2r  -> -3..4
-    z = (a.astype(np.float32) - np.float32(alpha)).astype(np.float32)
+    z = (a.astype(np.float32) - np.asarray(alpha, np.float32)[..., None]).astype(np.float32)
+    if np.ndim(alpha) == 0:
+        z = z.reshape(-1)
     return (kaiser(b, r, z) * sinc32(z)).astype(np.float32)
 
 
 def hickscoeffs(dz, dy, dx, z0, y0, x0, nz, ny, nx, z, y, x, fs_index=1, zstagger=0.0, ystagger=0.0, xstagger=0.0):
     """3D Kaiser-windowed sinc taps, src/spacetime.jl:867-925.  dz,dy,dx are the element type
-    (float32 here); z,y,x float64 positions.  Returns a SourcePoint32 array."""
+    (float32 here); z,y,x float64 positions.  Returns a SourcePoint32 array.  All points at once:
+    per point the 8 x 8 x 8 taps in the reference's loop order (iz fastest, then iy, ix), with
+    the reference's float32 products and free-surface image terms."""
     z, y, x = (np.atleast_1d(np.asarray(v, np.float64)) for v in (z, y, x))
     hb, hr = np.float32(6.42), 4
     hl = 2 * hr
     dzd, dyd, dxd = float(np.float32(dz)), float(np.float32(dy)), float(np.float32(dx))
-    iz_all, iy_all, ix_all, ir_all, c_all = [], [], [], [], []
-    for i in range(len(z)):
-        iz_mid = int(np.floor((z[i] - z0) / dzd)) + 1
-        iy_mid = int(np.floor((y[i] - y0) / dyd)) + 1
-        ix_mid = int(np.floor((x[i] - x0) / dxd)) + 1
-        az = np.float32((z[i] - (z0 + (iz_mid - 1 + zstagger) * dzd)) / dzd)
-        ay = np.float32((y[i] - (y0 + (iy_mid - 1 + ystagger) * dyd)) / dyd)
-        ax = np.float32((x[i] - (x0 + (ix_mid - 1 + xstagger) * dxd)) / dxd)
-        fz, fy, fx = _hicks_1d(hb, hr, az), _hicks_1d(hb, hr, ay), _hicks_1d(hb, hr, ax)
-        # d[i,j,k] = fz[i]*fy[j]*fx[k] (float32); Julia column-major (iz fastest)
-        ci = (fz[:, None, None] * fy[None, :, None]).astype(np.float32)[:, :, :] * fx[None, None, :]
-        ci = ci.astype(np.float32)
-        izs = iz_mid - hr + np.arange(1, hl + 1)
-        # reflect source terms above the free surface (src/spacetime.jl:905-917)
-        if izs[0] == fs_index - 3:
-            ci[4] -= ci[2]
-            ci[5] -= ci[1]
-            ci[6] -= ci[0]
-        if izs[0] == fs_index - 2:
-            ci[3] -= ci[1]
-            ci[4] -= ci[0]
-        if izs[0] == fs_index - 1:
-            ci[2] -= ci[0]
-        iys = iy_mid - hr + np.arange(1, hl + 1)
-        ixs = ix_mid - hr + np.arange(1, hl + 1)
-        # Julia loop order for ix, iy, iz (iz fastest) -> flatten in Fortran order
-        IZ, IY, IX = np.meshgrid(izs, iys, ixs, indexing="ij")
-        iz_all.append(IZ.ravel(order="F"))
-        iy_all.append(IY.ravel(order="F"))
-        ix_all.append(IX.ravel(order="F"))
-        ir_all.append(np.full(hl ** 3, i + 1))
-        c_all.append(ci.ravel(order="F"))
-    return _finish(np.concatenate(iz_all), np.concatenate(iy_all), np.concatenate(ix_all), np.concatenate(ir_all),
-                   np.concatenate(c_all).astype(np.float32), nz, ny, nx)
+    npt = len(z)
+    iz_mid = np.floor((z - z0) / dzd).astype(np.int64) + 1
+    iy_mid = np.floor((y - y0) / dyd).astype(np.int64) + 1
+    ix_mid = np.floor((x - x0) / dxd).astype(np.int64) + 1
+    az = ((z - (z0 + (iz_mid - 1 + zstagger) * dzd)) / dzd).astype(np.float32)
+    ay = ((y - (y0 + (iy_mid - 1 + ystagger) * dyd)) / dyd).astype(np.float32)
+    ax = ((x - (x0 + (ix_mid - 1 + xstagger) * dxd)) / dxd).astype(np.float32)
+    fz, fy, fx = _hicks_1d(hb, hr, az), _hicks_1d(hb, hr, ay), _hicks_1d(hb, hr, ax)  # (npt, 8)
+    # d[p,i,j,k] = fz[p,i]*fy[p,j]*fx[p,k] in float32
+    ci = (fz[:, :, None, None] * fy[:, None, :, None]).astype(np.float32) * fx[:, None, None, :]
+    ci = ci.astype(np.float32)
+    # reflect source terms above the free surface (src/spacetime.jl:905-917)
+    iz0 = iz_mid - hr + 1
+    m = iz0 == fs_index - 3
+    ci[m, 4] -= ci[m, 2]
+    ci[m, 5] -= ci[m, 1]
+    ci[m, 6] -= ci[m, 0]
+    m = iz0 == fs_index - 2
+    ci[m, 3] -= ci[m, 1]
+    ci[m, 4] -= ci[m, 0]
+    m = iz0 == fs_index - 1
+    ci[m, 2] -= ci[m, 0]
+    off = np.arange(1, hl + 1) - hr
+    shape = (npt, hl, hl, hl)
+    IZ = np.broadcast_to((iz_mid[:, None] + off[None, :])[:, :, None, None], shape)
+    IY = np.broadcast_to((iy_mid[:, None] + off[None, :])[:, None, :, None], shape)
+    IX = np.broadcast_to((ix_mid[:, None] + off[None, :])[:, None, None, :], shape)
+
+    def order(a):  # per point, iz fastest
+        return np.ascontiguousarray(a.transpose(0, 3, 2, 1)).ravel()
+
+    ir = np.repeat(np.arange(1, npt + 1), hl ** 3)
+    return _finish(order(IZ), order(IY), order(IX), ir, order(ci), nz, ny, nx)
 
 
 def linearcoeffs(dz, dy, dx, z0, y0, x0, nz, ny, nx, z, y, x, fs_index=1):
-    """3D trilinear taps, src/spacetime.jl:967-1013."""
+    """3D trilinear taps, src/spacetime.jl:967-1013, all points at once: per point the 8 taps in
+    the reference's loop order (iz fastest, then iy, ix), weights wx*wy*wz in float64 rounded to
+    float32, the free-surface row zeroed."""
     z, y, x = (np.atleast_1d(np.asarray(v, np.float64)) for v in (z, y, x))
     dzd, dyd, dxd = float(np.float32(dz)), float(np.float32(dy)), float(np.float32(dx))
-    iz_all, iy_all, ix_all, ir_all, c_all = [], [], [], [], []
-    for i in range(len(z)):
-        kx = int(np.floor((x[i] - x0) / dxd)) + 1
-        ky = int(np.floor((y[i] - y0) / dyd)) + 1
-        kz = int(np.floor((z[i] - z0) / dzd)) + 1
-        rx = (x[i] - (x0 + dxd * (kx - 1))) / dxd
-        ry = (y[i] - (y0 + dyd * (ky - 1))) / dyd
-        rz = (z[i] - (z0 + dzd * (kz - 1))) / dzd
-        c = np.empty((2, 2, 2))  # [iz, iy, ix]
-        for a in range(2):
-            for bb in range(2):
-                for cc in range(2):
-                    wz = rz if a else 1.0 - rz
-                    wy = ry if bb else 1.0 - ry
-                    wx = rx if cc else 1.0 - rx
-                    c[a, bb, cc] = wx * wy * wz
-        c = c.astype(np.float32)
-        izs = np.array([kz, kz + 1])
-        if izs[0] == fs_index:  # the free surface row gets no weight (src/spacetime.jl:1000-1005)
-            c[0, :, :] = 0
-        IZ, IY, IX = np.meshgrid(izs, [ky, ky + 1], [kx, kx + 1], indexing="ij")
-        iz_all.append(IZ.ravel(order="F"))
-        iy_all.append(IY.ravel(order="F"))
-        ix_all.append(IX.ravel(order="F"))
-        ir_all.append(np.full(8, i + 1))
-        c_all.append(c.ravel(order="F"))
-    return _finish(np.concatenate(iz_all), np.concatenate(iy_all), np.concatenate(ix_all), np.concatenate(ir_all),
-                   np.concatenate(c_all).astype(np.float32), nz, ny, nx)
+    kx = np.floor((x - x0) / dxd).astype(np.int64) + 1
+    ky = np.floor((y - y0) / dyd).astype(np.int64) + 1
+    kz = np.floor((z - z0) / dzd).astype(np.int64) + 1
+    rx = (x - (x0 + dxd * (kx - 1))) / dxd
+    ry = (y - (y0 + dyd * (ky - 1))) / dyd
+    rz = (z - (z0 + dzd * (kz - 1))) / dzd
+    npt = len(z)
+    iz, iy, ix = (np.empty((npt, 8), np.int64) for _ in range(3))
+    c = np.empty((npt, 8), np.float32)
+    fsrow = kz == fs_index  # the free surface row gets no weight (src/spacetime.jl:1000-1005)
+    for cc in range(2):
+        wx = rx if cc else 1.0 - rx
+        for bb in range(2):
+            wy = ry if bb else 1.0 - ry
+            for a in range(2):
+                wz = rz if a else 1.0 - rz
+                j = a + 2 * bb + 4 * cc
+                iz[:, j], iy[:, j], ix[:, j] = kz + a, ky + bb, kx + cc
+                cj = (wx * wy * wz).astype(np.float32)
+                if a == 0:
+                    cj[fsrow] = 0
+                c[:, j] = cj
+    ir = np.repeat(np.arange(1, npt + 1), 8)
+    return _finish(iz.ravel(), iy.ravel(), ix.ravel(), ir, c.ravel(), nz, ny, nx)
 
 
 def _finish(iz, iy, ix, ir, c, nz, ny, nx):
