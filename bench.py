@@ -94,7 +94,39 @@ def build_model(torch, physics, n, d, dt, seed, dev):
     return m
 
 
-def run_gpu(physics, n, steps, warmup, seed=3, tune=False):
+def _time_taps(torch, w, prop, physics, n, d, steps, warmup, stream):
+    """The same handle's modelling loop: a Hicks source half a cell off the grid at 20 m depth
+    in the middle, n x n receivers on the nodes of the row at 2d depth (one unit tap each), and
+    per step TimeStepInjectExtract (the step, then the registered injection and extraction
+    kernels on its stream).  HIP-event and wall time per step."""
+    f32 = np.float32
+    nt = warmup + steps
+    mid = (n / 2 - 0.5) * d
+    src = w.hickscoeffs(f32(d), f32(d), f32(d), 0.0, 0.0, 0.0, n, n, n, [20.0], [mid], [mid])
+    y, x = np.meshgrid(np.arange(n) * d, np.arange(n) * d, indexing="ij")
+    rcv = w.linearcoeffs(f32(d), f32(d), f32(d), 0.0, 0.0, 0.0, n, n, n, np.full(n * n, 2 * d), y.ravel(),
+                         x.ravel())
+    wavelet = w.causal_ricker(15.0, 1e-3 * np.arange(nt))[None, :]
+    w.set_sources(prop, w.source_blocking(src, 8), wavelet, fields=1 if physics == "iso" else 3)
+    w.set_receivers(prop, w.receiver_blocking(rcv, 8), nt)
+    for it in range(1, warmup + 1):
+        w.timestep_inject_extract(prop, it)
+    w.synchronize(prop)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    ev0.record(stream)
+    for it in range(warmup + 1, nt + 1):
+        w.timestep_inject_extract(prop, it)
+    ev1.record(stream)
+    w.synchronize(prop)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    return {"ms_per_step": wall * 1e3 / steps, "gpu_ms_per_step": ev0.elapsed_time(ev1) / steps,
+            "receivers": n * n, "receiver_taps": int(len(rcv)), "source_taps": int(len(src))}
+
+
+def run_gpu(physics, n, steps, warmup, seed=3, tune=False, with_taps=False):
     import torch
     import wavefd_amd as w
 
@@ -153,6 +185,7 @@ def run_gpu(physics, n, steps, warmup, seed=3, tune=False):
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     ms_events = ev0.elapsed_time(ev1)
+    taps = _time_taps(torch, w, prop, physics, n, d, steps, warmup, stream) if with_taps else None
     pc = torch.empty((n, n, n), device=dev)
     hip.hipMemcpy(pc.data_ptr(), w.device_ptr(prop, "pCur"), pc.numel() * 4, 3)
     finite = bool(torch.isfinite(pc).all())
@@ -167,7 +200,7 @@ def run_gpu(physics, n, steps, warmup, seed=3, tune=False):
         "ms_per_step": ms_per_step, "kernel_ms": kernel_ms,
         "mcells_s": cells * steps / wall / 1e6,
         "achieved_gbs": cells * BYTES_PER_CELL[physics] / (kernel_ms * 1e-3) / 1e9,
-        "finite": finite, "launch": launch, "autotune": tuned,
+        "finite": finite, "launch": launch, "autotune": tuned, "taps": taps,
     }
 
 
@@ -689,7 +722,7 @@ def main():
               "build has no multi-GPU path; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    res = run_gpu(args.physics, args.n, args.steps, args.warmup, tune=args.autotune)
+    res = run_gpu(args.physics, args.n, args.steps, args.warmup, tune=args.autotune, with_taps=True)
     extra = []
     if args.extra:
         for ph, n in (("iso", 256), ("vti", 512)):
@@ -733,6 +766,12 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if res["taps"]:
+        # the step with registered source injection and receiver extraction, timed separately
+        t = res["taps"]
+        line["step_inject_extract"] = dict(t, ms_per_step=round(t["ms_per_step"], 4),
+                                           gpu_ms_per_step=round(t["gpu_ms_per_step"], 4),
+                                           overhead_vs_step=round(t["gpu_ms_per_step"] / res["kernel_ms"] - 1, 4))
     if res["autotune"]:
         line["autotune"] = res["autotune"]
     if extra:
