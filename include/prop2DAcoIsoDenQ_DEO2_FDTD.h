@@ -55,6 +55,17 @@ float *Prop2DAcoIsoDenQ_DEO2_FDTD_GetDevicePtr(void *p, const char *name);
 /* 1 (default): the fused single-pass step kernel; 0: the two passes (sandwich, then D- and
  * the update); the results are the same */
 void Prop2DAcoIsoDenQ_DEO2_FDTD_SetFusedStep(void *p, long fused);
+/* the Born loops' per-step pairs as one step; no counterpart in the reference (its callers
+ * make the two calls).  Same results as the pairs, on the same arguments:
+ *   TimeStepSaveDP(p, DP)                   = TimeStep; ScaleSpatialDerivatives, with the
+ *                                                 scaled pSpace written to DP
+ *   TimeStepBornInjection_V(p, dV, DP)      = TimeStep; ForwardBornInjection_V(dV, DP)
+ *   TimeStepAdjointBornAccumulation_V(p, g, DP)
+ *                                               = TimeStep; AdjointBornAccumulation_V(g, DP)
+ * pSpace is not part of the result.  Arrays: host or device.  0 on success. */
+long Prop2DAcoIsoDenQ_DEO2_FDTD_TimeStepSaveDP(void *p, float *DP);
+long Prop2DAcoIsoDenQ_DEO2_FDTD_TimeStepBornInjection_V(void *p, float *dVel, float *DP);
+long Prop2DAcoIsoDenQ_DEO2_FDTD_TimeStepAdjointBornAccumulation_V(void *p, float *dVel, float *DP);
 /* the handle's HIP stream (device-mode calls are ordered on it) and a wait for it */
 void *Prop2DAcoIsoDenQ_DEO2_FDTD_GetStream(void *p);
 void Prop2DAcoIsoDenQ_DEO2_FDTD_Synchronize(void *p);

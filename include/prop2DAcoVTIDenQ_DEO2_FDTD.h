@@ -71,6 +71,17 @@ float *Prop2DAcoVTIDenQ_DEO2_FDTD_GetDevicePtr(void *p, const char *name);
 /* 1 (default): the fused single-pass step kernel; 0: the two passes (sandwich, then D- and
  * the update); the results are the same */
 void Prop2DAcoVTIDenQ_DEO2_FDTD_SetFusedStep(void *p, long fused);
+/* the Born loops' per-step pairs as one step; no counterpart in the reference (its callers
+ * make the two calls).  Same results as the pairs, on the same arguments:
+ *   TimeStepSaveDP(p, DP, DM)                   = TimeStep; ScaleSpatialDerivatives, with the
+ *                                                 scaled pSpace (mSpace) written to DP (DM)
+ *   TimeStepBornInjection_V(p, dV, DP, DM)      = TimeStep; ForwardBornInjection_V(dV, DP, DM)
+ *   TimeStepAdjointBornAccumulation_V(p, g, DP, DM)
+ *                                               = TimeStep; AdjointBornAccumulation_V(g, DP, DM)
+ * pSpace/mSpace are not part of the result.  Arrays: host or device.  0 on success. */
+long Prop2DAcoVTIDenQ_DEO2_FDTD_TimeStepSaveDP(void *p, float *DP, float *DM);
+long Prop2DAcoVTIDenQ_DEO2_FDTD_TimeStepBornInjection_V(void *p, float *dVel, float *DP, float *DM);
+long Prop2DAcoVTIDenQ_DEO2_FDTD_TimeStepAdjointBornAccumulation_V(void *p, float *dVel, float *DP, float *DM);
 /* the handle's HIP stream (device-mode calls are ordered on it) and a wait for it */
 void *Prop2DAcoVTIDenQ_DEO2_FDTD_GetStream(void *p);
 void Prop2DAcoVTIDenQ_DEO2_FDTD_Synchronize(void *p);

@@ -180,6 +180,9 @@ _ISO2D = {
     "GetStream": (_P, [_P]),
     "Synchronize": (None, [_P]),
     "SetFusedStep": (None, [_P, _L]),
+    "TimeStepSaveDP": (_L, [_P, _FP]),
+    "TimeStepBornInjection_V": (_L, [_P, _FP, _FP]),
+    "TimeStepAdjointBornAccumulation_V": (_L, [_P, _FP, _FP]),
     "LastError": (ctypes.c_char_p, []),
     "ClearError": (None, []),
 }
@@ -217,6 +220,9 @@ _ANISO2D = {
     "GetStream": (_P, [_P]),
     "Synchronize": (None, [_P]),
     "SetFusedStep": (None, [_P, _L]),
+    "TimeStepSaveDP": (_L, [_P, _FP, _FP]),
+    "TimeStepBornInjection_V": (_L, [_P, _FP, _FP, _FP]),
+    "TimeStepAdjointBornAccumulation_V": (_L, [_P, _FP, _FP, _FP]),
     "LastError": (ctypes.c_char_p, []),
     "ClearError": (None, []),
 }

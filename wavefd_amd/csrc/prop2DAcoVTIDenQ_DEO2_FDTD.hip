@@ -97,6 +97,16 @@ float *Prop2DAcoVTIDenQ_DEO2_FDTD_GetDevicePtr(void *p, const char *name) {
     return (i >= 0 && P2(p)->f[i].used) ? P2(p)->f[i].d : nullptr;
 }
 void Prop2DAcoVTIDenQ_DEO2_FDTD_SetFusedStep(void *p, long fused) { P2(p)->fused = fused ? 1 : 0; }
+// additive: the Born loops' per-step pairs as one step (same results as the two calls)
+long Prop2DAcoVTIDenQ_DEO2_FDTD_TimeStepSaveDP(void *p, float *DP, float *DM) {
+    return wfd::born_step(P2(p), wfd::EPI_SAVE_DP, DP, DM, nullptr) ? 0 : -1;
+}
+long Prop2DAcoVTIDenQ_DEO2_FDTD_TimeStepBornInjection_V(void *p, float *dVel, float *DP, float *DM) {
+    return wfd::born_step(P2(p), wfd::EPI_BORN_INJ, dVel, DP, DM) ? 0 : -1;
+}
+long Prop2DAcoVTIDenQ_DEO2_FDTD_TimeStepAdjointBornAccumulation_V(void *p, float *dVel, float *DP, float *DM) {
+    return wfd::born_step(P2(p), wfd::EPI_IMAGE, dVel, DP, DM) ? 0 : -1;
+}
 void *Prop2DAcoVTIDenQ_DEO2_FDTD_GetStream(void *p) { return (void *)P2(p)->stream; }
 void Prop2DAcoVTIDenQ_DEO2_FDTD_Synchronize(void *p) {
     if (hipStreamSynchronize(P2(p)->stream) != hipSuccess) wfd::set_error("Synchronize: stream error");

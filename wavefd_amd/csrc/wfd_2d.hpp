@@ -429,6 +429,25 @@ __global__ void __launch_bounds__(256) k2_dplus_coef(Grid g, float4 cc, const fl
 
 // ---- operations ---------------------------------------------------------------------------
 
+// the fused 2D step with a Born epilogue (EPI_SAVE_DP, EPI_BORN_INJ, EPI_IMAGE; arrays as in
+// launch_fused2)
+template <int PHYS>
+static inline bool launch_fused2_epi(Prop *p, int epi, float *e0, float *e1, float *e2) {
+    return epi == EPI_SAVE_DP    ? launch_fused2<PHYS, EPI_SAVE_DP>(p, false, e0, e1, e2)
+           : epi == EPI_BORN_INJ ? launch_fused2<PHYS, EPI_BORN_INJ>(p, false, e0, e1, e2)
+                                 : launch_fused2<PHYS, EPI_IMAGE>(p, false, e0, e1, e2);
+}
+
+static inline bool step_2d_epi(Prop *p, int epi, float *e0, float *e1, float *e2) {
+    if (p->nx * p->nz >= (1L << 31)) {
+        set_error("2D fused step: fewer than 2^31 cells");
+        return false;
+    }
+    return p->phys == ISO   ? launch_fused2_epi<ISO>(p, epi, e0, e1, e2)
+           : p->phys == VTI ? launch_fused2_epi<VTI>(p, epi, e0, e1, e2)
+                            : launch_fused2_epi<TTI>(p, epi, e0, e1, e2);
+}
+
 static inline bool step_2d(Prop *p, bool linear) {
     const Grid &g = p->g;
     Field *f = p->f;

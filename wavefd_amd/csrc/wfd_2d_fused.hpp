@@ -20,6 +20,13 @@
 // Semantics match the two-pass kernels (wfd_2d.hpp) exactly: temporaries are 0 outside the
 // update region, the free surface uses the odd mirror for D+ and the even mirror for D-, the x
 // derivative vanishes at kz = 0 and L = 0 there, expression order unchanged.
+//
+// Epilogues (the Born loops' per-step calls, as in the 3D kernel): SAVE_DP writes DP = L v^2/b
+// (DM) instead of pSpace (ScaleSpatialDerivatives); BORN_INJ adds the V source 2 dt^2 dV/v DP
+// (DM) to the new field (src/prop2DAcoIsoDenQ_DEO2_FDTD.h ForwardBornInjection_V, and the VTI/TTI
+// twins); IMAGE adds 2b/v^3 (DP p + DM m) of this step's input fields to g
+// (AdjointBornAccumulation_V).  Every cell gets the pointwise kernel's expression, annulus
+// included, where L = 0 and the new field keeps pOld's value.
 #pragma once
 #include "wfd_fused.hpp"
 #include "wfd_kernels.hpp"
@@ -38,6 +45,11 @@ struct Fused2Args {
     const float *pCur, *mCur;
     float *pOld, *mOld, *pSpace, *mSpace;
     const float *v, *b, *q, *eps, *eta, *f, *sT, *cT;
+    // epilogue operands: SAVE_DP: DP, DM out (e_dp, e_dm); BORN_INJ: dV, DP, DM in (e_dv,
+    // e_ip, e_im); IMAGE: g in/out (e_dp), DP, DM in (e_ip, e_im)
+    float dt;
+    float *e_dp, *e_dm;
+    const float *e_dv, *e_ip, *e_im;
 };
 
 template <int PHYS>
@@ -67,12 +79,18 @@ __device__ __forceinline__ float sten8v(const float *a) {
     return C1 * (-a[3] + a[4]) + C2 * (-a[2] + a[5]) + C3 * (-a[1] + a[6]) + C4 * (-a[0] + a[7]);
 }
 
-template <int PHYS, bool FST>
+template <int PHYS, int EPI, bool FST>
 __device__ __forceinline__ void fused2_body(const Fused2Args<PHYS> &A, Fused2Lds<PHYS> &S, int zt, int xc) {
     constexpr int NF = Fused2Lds<PHYS>::NF;
     constexpr int NQ = PHYS == TTI ? 2 : 1;  // x-queues per field
     constexpr int NM = PHYS == ISO ? 1 : (PHYS == VTI ? 4 : 6);
-    constexpr int NO = 2 + NF;  // v, q, pOld (, mOld); b comes with the sandwich's material
+    // output operands: v, q, pOld (, mOld) (b comes with the sandwich's material), then for the
+    // epilogues (SAVE_DP) b, (BORN_INJ) dV, DP (, DM) or (IMAGE) b, g, DP (, DM)
+    constexpr int NEX = EPI == EPI_NONE ? 0 : (EPI == EPI_SAVE_DP ? 1 : (EPI == EPI_BORN_INJ ? 1 + NF : 2 + NF));
+    constexpr int IE = 2 + NF;  // first epilogue operand
+    constexpr int NO = IE + NEX;
+    static_assert(EPI == EPI_NONE || EPI == EPI_SAVE_DP || EPI == EPI_BORN_INJ || EPI == EPI_IMAGE,
+                  "2D epilogues: none, save DP, Born injection V, imaging V");
     const Grid &g = A.g;
     const int nx = (int)g.nx, nz = (int)g.nz;
     const int t = threadIdx.x;
@@ -110,7 +128,23 @@ __device__ __forceinline__ void fused2_body(const Fused2Args<PHYS> &A, Fused2Lds
     if (has_b) boff = col(z0 - 7 + bent, b_sign, b_ok);
     const float *fld[2] = {A.pCur, A.mCur};
     const float *mat[6] = {A.b, A.eps, A.eta, A.f, A.sT, A.cT};
-    const float *oin[4] = {A.v, A.q, A.pOld, A.mOld};
+    const float *oin[NO];
+    oin[0] = A.v;
+    oin[1] = A.q;
+    oin[2] = A.pOld;
+    if constexpr (NF == 2) oin[3] = A.mOld;
+    if constexpr (EPI == EPI_SAVE_DP) oin[IE] = A.b;
+    if constexpr (EPI == EPI_BORN_INJ) {
+        oin[IE] = A.e_dv;
+        oin[IE + 1] = A.e_ip;
+        if constexpr (NF == 2) oin[IE + 2] = A.e_im;
+    }
+    if constexpr (EPI == EPI_IMAGE) {
+        oin[IE] = A.b;
+        oin[IE + 1] = A.e_dp;
+        oin[IE + 2] = A.e_ip;
+        if constexpr (NF == 2) oin[IE + 3] = A.e_im;
+    }
     const long nzl = nz;
     auto ld = [&](const float *a, int xx, int off) -> float {
         const int xcl = xx < 0 ? 0 : (xx >= nx ? nx - 1 : xx);
@@ -233,16 +267,22 @@ __device__ __forceinline__ void fused2_body(const Fused2Args<PHYS> &A, Fused2Lds
         if (x < xs || x >= xe || !out_thread) return;
         const bool upd = x >= 4 && x < nx - 4 && zU;
         const long k0 = (long)x * nzl + ooff;
-        if (!upd) {
-            if (A.pSpace) {
-                A.pSpace[k0] = 0.0f;
-                if constexpr (NF == 2) A.mSpace[k0] = 0.0f;
+        if constexpr (EPI == EPI_NONE) {
+            if (!upd) {
+                if (A.pSpace) {
+                    A.pSpace[k0] = 0.0f;
+                    if constexpr (NF == 2) A.mSpace[k0] = 0.0f;
+                }
+                return;
             }
-            return;
         }
         float L[NF];
 #pragma unroll
         for (int fi = 0; fi < NF; fi++) {
+            if (!upd) {
+                L[fi] = 0.0f;
+                continue;
+            }
             if (FST && z == 0) {
                 L[fi] = 0.0f + 0.0f;
                 continue;
@@ -275,13 +315,55 @@ __device__ __forceinline__ void fused2_body(const Fused2Args<PHYS> &A, Fused2Lds
             }
         }
         const float V = oc[0];
-        const float dt2V2_B = A.dt2 * V * V / bq[sr];
-        const float qk = oc[1];
-        A.pOld[k0] = leapfrog(dt2V2_B, L[0], qk, pq[0][0], oc[2]);
-        if (A.pSpace) A.pSpace[k0] = L[0];
-        if constexpr (NF == 2) {
-            A.mOld[k0] = leapfrog(dt2V2_B, L[1], qk, pq[1][0], oc[3]);
-            if (A.mSpace) A.mSpace[k0] = L[1];
+        // the new field: updated inside the region, pOld's value in the annulus
+        float pn_[NF];
+        if (upd) {
+            const float dt2V2_B = A.dt2 * V * V / bq[sr];
+            const float qk = oc[1];
+            pn_[0] = leapfrog(dt2V2_B, L[0], qk, pq[0][0], oc[2]);
+            if constexpr (NF == 2) pn_[1] = leapfrog(dt2V2_B, L[1], qk, pq[1][0], oc[3]);
+        } else {
+#pragma unroll
+            for (int fi = 0; fi < NF; fi++) pn_[fi] = oc[2 + fi];
+        }
+        if constexpr (EPI == EPI_NONE) {
+            A.pOld[k0] = pn_[0];
+            if (A.pSpace) A.pSpace[k0] = L[0];
+            if constexpr (NF == 2) {
+                A.mOld[k0] = pn_[1];
+                if (A.mSpace) A.mSpace[k0] = L[1];
+            }
+        } else if constexpr (EPI == EPI_SAVE_DP) {
+            if (upd) {
+                A.pOld[k0] = pn_[0];
+                if constexpr (NF == 2) A.mOld[k0] = pn_[1];
+            }
+            const float sc = V * V / oc[IE];  // ScaleSpatialDerivatives: pSpace *= v^2/b
+            A.e_dp[k0] = L[0] * sc;
+            if constexpr (NF == 2) A.e_dm[k0] = L[1] * sc;
+        } else if constexpr (EPI == EPI_BORN_INJ) {
+            const float factor = 2 * A.dt * A.dt * oc[IE] / V;
+            pn_[0] += factor * oc[IE + 1];
+            A.pOld[k0] = pn_[0];
+            if constexpr (NF == 2) {
+                pn_[1] += factor * oc[IE + 2];
+                A.mOld[k0] = pn_[1];
+            }
+        } else {  // EPI_IMAGE: g += 2b/v^3 (DP p + DM m) with this step's input fields
+            if (upd) {
+                A.pOld[k0] = pn_[0];
+                if constexpr (NF == 2) A.mOld[k0] = pn_[1];
+            }
+            const float B = oc[IE];
+            float gk = oc[IE + 1];
+            if constexpr (NF == 1) {
+                const float factorV = +2 * B / (V * V * V);
+                gk += factorV * oc[IE + 2] * pq[0][0];
+            } else {
+                const float factor = 2 * B / (V * V * V);
+                gk += factor * (oc[IE + 2] * pq[0][0] + oc[IE + 3] * pq[1][0]);
+            }
+            A.e_dp[k0] = gk;
         }
     };
 
@@ -294,16 +376,16 @@ __device__ __forceinline__ void fused2_body(const Fused2Args<PHYS> &A, Fused2Lds
     }
 }
 
-template <int PHYS>
+template <int PHYS, int EPI>
 __global__ void __launch_bounds__(F2W) k2_fused_step(Fused2Args<PHYS> A) {
     __shared__ Fused2Lds<PHYS> S;
     const int bid = (int)blockIdx.x;
     const int zt = bid % A.nzt, xc = bid / A.nzt;
     if (xc >= A.nxc) return;
     if (A.g.fs && zt == 0)
-        fused2_body<PHYS, true>(A, S, zt, xc);
+        fused2_body<PHYS, EPI, true>(A, S, zt, xc);
     else
-        fused2_body<PHYS, false>(A, S, zt, xc);
+        fused2_body<PHYS, EPI, false>(A, S, zt, xc);
 }
 
 // x-planes per chunk: enough workgroups for about eight per CU, chunks of at least 32 planes
@@ -316,12 +398,22 @@ static inline int fused2_cx(const Prop *p, int nzt) {
     return (int)((p->nx + nxc - 1) / nxc);
 }
 
-template <int PHYS>
-static inline bool launch_fused2(Prop *p, bool linear) {
+// e0, e1, e2: the epilogue's arrays (device): SAVE_DP (DP, DM, -), BORN_INJ (dV, DP, DM),
+// IMAGE (g, DP, DM); the epilogues leave pSpace/mSpace as they were
+template <int PHYS, int EPI = EPI_NONE>
+static inline bool launch_fused2(Prop *p, bool linear, float *e0 = nullptr, float *e1 = nullptr,
+                                 float *e2 = nullptr) {
     const Field *f = p->f;
     Fused2Args<PHYS> A;
     A.g = p->g;
     A.dt2 = p->dt * p->dt;
+    A.dt = p->dt;
+    A.e_dp = EPI == EPI_SAVE_DP || EPI == EPI_IMAGE ? e0 : nullptr;
+    A.e_dm = EPI == EPI_SAVE_DP ? e1 : nullptr;
+    A.e_dv = EPI == EPI_BORN_INJ ? e0 : nullptr;
+    A.e_ip = EPI == EPI_BORN_INJ || EPI == EPI_IMAGE ? e1 : nullptr;
+    A.e_im = EPI == EPI_BORN_INJ || EPI == EPI_IMAGE ? e2 : nullptr;
+    if (EPI != EPI_NONE) linear = true;  // pSpace/mSpace untouched
     A.nzt = (int)((p->nz + F2O - 1) / F2O);
     A.cx = fused2_cx(p, A.nzt);
     A.nxc = (int)((p->nx + A.cx - 1) / A.cx);
@@ -341,7 +433,7 @@ static inline bool launch_fused2(Prop *p, bool linear) {
     A.cT = PHYS == TTI ? f[F_CT].d : nullptr;
     if (PHYS == ISO) A.mCur = A.pCur, A.mOld = A.pOld;  // never read: one wavefield
     const long nwg = (long)A.nzt * A.nxc;
-    k2_fused_step<PHYS><<<dim3((unsigned)nwg), dim3(F2W), 0, p->stream>>>(A);
+    k2_fused_step<PHYS, EPI><<<dim3((unsigned)nwg), dim3(F2W), 0, p->stream>>>(A);
     WFD_LAUNCHED();
     return true;
 }

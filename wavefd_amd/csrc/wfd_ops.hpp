@@ -466,7 +466,8 @@ static inline bool born_step(Prop *p, int epi, float *a0, float *a1, float *a2) 
     }
     float *e0 = a[0].dev, *e1 = na > 1 ? a[1].dev : nullptr, *e2 = na > 2 ? a[2].dev : nullptr;
     if (p->fused) {
-        bool ok = epi == EPI_SAVE_DP    ? step_fused_epi<EPI_SAVE_DP>(p, false, e0, e1, nullptr)
+        bool ok = p->dim == 2             ? step_2d_epi(p, epi, e0, e1, e2)
+                  : epi == EPI_SAVE_DP    ? step_fused_epi<EPI_SAVE_DP>(p, false, e0, e1, nullptr)
                   : epi == EPI_BORN_INJ ? step_fused_epi<EPI_BORN_INJ>(p, false, e0, e1, e2)
                                         : step_fused_epi<EPI_IMAGE>(p, false, e0, e1, e2);
         if (!ok) return false;
@@ -481,6 +482,8 @@ static inline bool born_step(Prop *p, int epi, float *a0, float *a1, float *a2) 
             if (two)
                 k_vti_inj_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, f[F_V].d, e0, e1, e2, f[F_PCUR].d,
                                                                        f[F_MCUR].d);
+            else if (p->dim == 2)  // the 2D text's V source: p += 2 dt^2 dV / v DP
+                k2_iso_inj_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, f[F_V].d, e0, e1, f[F_PCUR].d);
             else
                 k_iso_inj_v<<<pw_grid(p->n), PW_BLOCK, 0, p->stream>>>(p->n, p->dt, f[F_V].d, f[F_B].d, e0, e1,
                                                                        f[F_PCUR].d);
