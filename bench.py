@@ -451,6 +451,10 @@ def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000), fused=Tr
             else "separate calls"}
 
 
+# the predictive snapshot store's settings: frames per segment, extrapolation order, keyframe bits
+PREDICTIVE = (48, 3, 16)
+
+
 def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
     """Learned snapshot codec (wavefd_amd/lcodec.py) on the bench's Iso model construction.
     Training: DP snapshots (every 3rd step) of an n^3 shot made by the propagator's fused SaveDP
@@ -542,18 +546,21 @@ def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
     ry, rx = np.meshgrid(np.linspace(30 * d, (n - 30) * d, 16), np.linspace(30 * d, (n - 30) * d, 16), indexing="ij")
     shot = born.BornShot((20.0, n * d / 2 + 3.0, n * d / 2 - 4.0), np.full(ry.size, 20.0), ry.ravel(), rx.ravel(),
                          st.causal_ricker(15.0, dt * np.arange(nt)))
-    imgs, times = {}, {}
-    for mode in ("exact", "plain", "learned"):
+    imgs, times, stored = {}, {}, {}
+    for mode in ("exact", "plain", "learned", "predictive"):
         img = ginsu.DeviceBuffer(4 * n ** 3)
         img.zero()
         t0 = time.perf_counter()
-        born.born_shot("iso", win, (d, d, d), dt, shot, {k: q.ptr for k, q in m2.items()}, dvb.ptr, img.ptr,
-                       nsponge=20, snapshot_codec=None if mode == "exact" else codec, learned=mode == "learned")
+        r = born.born_shot("iso", win, (d, d, d), dt, shot, {k: q.ptr for k, q in m2.items()}, dvb.ptr, img.ptr,
+                           nsponge=20, snapshot_codec=codec if mode in ("plain", "learned") else None,
+                           learned=mode == "learned", predictive=PREDICTIVE if mode == "predictive" else None)
         times[mode] = time.perf_counter() - t0
+        if r["stored_bytes"]:
+            stored[mode] = r["stored_bytes"]
         imgs[mode] = img.download(np.zeros(n ** 3, F32)).astype(np.float64)
         img.free()
     ref_n = np.linalg.norm(imgs["exact"])
-    gerr = {k: float(np.linalg.norm(imgs[k] - imgs["exact"]) / ref_n) for k in ("plain", "learned")}
+    gerr = {k: float(np.linalg.norm(imgs[k] - imgs["exact"]) / ref_n) for k in ("plain", "learned", "predictive")}
     for q in list(m1.values()) + list(m2.values()) + [dvb]:
         q.free()
     # rates on a big field
@@ -592,7 +599,10 @@ def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
                       "loss_last": float(losses[-50:].mean()), "generate_seconds": round(t_gen, 2)},
             "snapshot_rel_err": {"plain": round(float(np.mean(ep)), 4), "learned": round(float(np.mean(el)), 4),
                                  "snapshots": len(ep)},
-            "born_image_rel_err_vs_exact": {k: round(v, 4) for k, v in gerr.items()},
+            "born_image_rel_err_vs_exact": {k: round(v, 6) for k, v in gerr.items()},
+            "predictive": {"segment": PREDICTIVE[0], "order": PREDICTIVE[1], "keyframe_bits": PREDICTIVE[2],
+                           "stored_ratio_vs_exact": round(4 * n ** 3 * nt / stored["predictive"], 3),
+                           "residuals": "11-level block quantiser"},
             "born_seconds": {k: round(v, 2) for k, v in times.items()},
             "rates_GBps_512": {"encode_learned": round(gb / t_enc, 1), "decode_learned": round(gb / t_dec, 1),
                                "encode_plus_dequantize_plain": round(gb / t_plain, 1)},

@@ -34,6 +34,11 @@ long wavefd_lcodec_dequantize(const uint32_t *code, float *out, void *stream);  
 float wavefd_lcodec_norm(const uint32_t *code);  /* the code's largest block scale; -1 on error */
 /* half a quantisation step of every value's block, s / 10: |x - x_hat| is at most that */
 long wavefd_lcodec_halfstep(const uint32_t *code, float *out, void *stream);
+/* out = ca a + cb b + cc c over n floats (device; b and c may be NULL), evaluated in that order:
+ * the predictions, residuals and reconstructions of the Born driver's predictive snapshot store,
+ * the same call on the encoding and the decoding side; no counterpart in the reference */
+long wavefd_lcodec_combine(long n, float ca, const float *a, float cb, const float *b, float cc, const float *c,
+                           float *out, void *stream);
 
 /* the decoder network */
 long wavefd_lcodec_nparams(void);

@@ -598,12 +598,39 @@ void host_mask(std::vector<float> &mk) {
     // no biases: f(0) = 0 and f(a q) = a f(q) for a > 0
 }
 
+// out = ca a + cb b + cc c, b and c optional, evaluated in that order: the combinations of a
+// closed-loop temporal predictor (prediction 2 a - b, residual x - prediction, reconstruction
+// prediction + decoded residual), the same kernel on the encoding and the decoding side
+__global__ void k_combine(long n, float ca, const float *__restrict__ a, float cb, const float *__restrict__ b,
+                          float cc, const float *__restrict__ c, float *__restrict__ out) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float v = ca * a[i];
+        if (b) v = v + cb * b[i];
+        if (c) v = v + cc * c[i];
+        out[i] = v;
+    }
+}
+
 }  // namespace
 
 #pragma GCC visibility push(default)
 extern "C" {
 
 const char *wavefd_lcodec_LastError(void) { return g_err.c_str(); }
+
+long wavefd_lcodec_combine(long n, float ca, const float *a, float cb, const float *b, float cc, const float *c,
+                           float *out, void *stream) {
+    if (n < 0 || (n > 0 && (!a || !out))) {
+        g_err = "combine: bad arguments";
+        return -1;
+    }
+    if (n == 0) return 0;
+    const long blocks = std::min<long>((n + 255) / 256, 1L << 20);
+    k_combine<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(n, ca, a, cb, b, cc, c, out);
+    LC_TRY(hipGetLastError());
+    return 0;
+}
 
 long wavefd_lcodec_code_words(long n1, long n2, long n3) {
     if (n1 <= 0 || n2 <= 0 || n3 <= 0) return -1;

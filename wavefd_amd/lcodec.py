@@ -45,6 +45,13 @@ def dequantize(code, out, stream=None):
         raise RuntimeError(_err())
 
 
+def combine(n, ca, a, out, cb=0.0, b=None, cc=0.0, c=None, stream=None):
+    """out = ca a + cb b + cc c on n floats (device addresses; b, c optional), in that order"""
+    if _lib.load("lcodec").wavefd_lcodec_combine(int(n), float(ca), int(a), float(cb), None if b is None else int(b),
+                                                 float(cc), None if c is None else int(c), int(out), stream) != 0:
+        raise RuntimeError(_err())
+
+
 class LearnedCodec:
     """The quantiser with its learned decoder.  Parameters: per layer W[t][co][c] (27 x 16 x 16,
     t = (dx+1) 9 + (dy+1) 3 + (dz+1)) and 16 biases, three layers (conv1 reads channel 0 only,
