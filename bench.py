@@ -483,7 +483,9 @@ def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
         del m
         return out
 
-    def snapshots(mdl):
+    def snapshots(mdl, residual=False):
+        """DP every 3rd step; residual: instead the open-loop residual of a third-order temporal
+        extrapolation, DP_n - (3 DP_n-1 - 3 DP_n-2 + DP_n-3), what the predictive store codes"""
         p = w.Prop3DAcoIsoDenQ_DEO2_FDTD(nz=n, ny=n, nx=n, dz=d, dy=d, dx=d, dt=dt, nsponge=20, freesurface=True,
                                          freqQ=10.0, qMin=0.1, qInterior=100.0)
         w.coherence(p, "device")
@@ -493,18 +495,25 @@ def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
         src = st.hickscoeffs(F32(d), F32(d), F32(d), 0.0, 0.0, 0.0, n, n, n, [20.0], [n * d / 2 + 3.0],
                              [n * d / 2 - 4.0], fs_index=1)
         w.set_sources(p, [src], st.causal_ricker(15.0, dt * np.arange(nt)).reshape(1, -1).astype(F32))
-        dp = ginsu.DeviceBuffer(4 * n ** 3)
+        ring = [ginsu.DeviceBuffer(4 * n ** 3) for _ in range(4)]  # DP of the last 4 steps, newest first
+        pred = ginsu.DeviceBuffer(4 * n ** 3)
         out = []
         strm = p._lib.GetStream(p.p)
         for it in range(1, nt + 1):
-            p._lib.TimeStepSaveDP(p.p, dp.ptr)
+            p._lib.TimeStepSaveDP(p.p, ring[3].ptr)
             p._lib.InjectRegistered(p.p, it)
-            if it % 3 == 0:
+            ring = [ring[3]] + ring[:3]
+            if it % 3 == 0 and (not residual or it >= 4):
                 b = ginsu.DeviceBuffer(4 * n ** 3)
-                ginsu.copy(b.ptr, dp.ptr, 4 * n ** 3, strm)
+                if residual:
+                    lcodec.combine(n ** 3, 3.0, ring[1].ptr, pred.ptr, -3.0, ring[2].ptr, 1.0, ring[3].ptr, stream=strm)
+                    lcodec.combine(n ** 3, 1.0, ring[0].ptr, b.ptr, -1.0, pred.ptr, stream=strm)
+                else:
+                    ginsu.copy(b.ptr, ring[0].ptr, 4 * n ** 3, strm)
                 out.append(b)
         w.synchronize(p)
-        dp.free()
+        for q in ring + [pred]:
+            q.free()
         w.free(p)
         return out
 
@@ -517,6 +526,12 @@ def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
     losses = codec.train([b.ptr for b in train], shape, iters=iters, patch=32, batch=4, lr=1e-3, seed=seed)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
+    # a second decoder, trained on the residual frames the predictive store codes
+    rtrain = snapshots(m1, residual=True)
+    rcodec = lcodec.LearnedCodec(seed=seed + 1)
+    rlosses = rcodec.train([b.ptr for b in rtrain], shape, iters=iters, patch=32, batch=4, lr=1e-3, seed=seed)
+    for b in rtrain:
+        b.free()
     for b in train:
         b.free()
     # snapshot errors on the second model
@@ -547,20 +562,22 @@ def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
     shot = born.BornShot((20.0, n * d / 2 + 3.0, n * d / 2 - 4.0), np.full(ry.size, 20.0), ry.ravel(), rx.ravel(),
                          st.causal_ricker(15.0, dt * np.arange(nt)))
     imgs, times, stored = {}, {}, {}
-    for mode in ("exact", "plain", "learned", "predictive"):
+    for mode in ("exact", "plain", "learned", "predictive", "predictive_learned"):
         img = ginsu.DeviceBuffer(4 * n ** 3)
         img.zero()
         t0 = time.perf_counter()
         r = born.born_shot("iso", win, (d, d, d), dt, shot, {k: q.ptr for k, q in m2.items()}, dvb.ptr, img.ptr,
                            nsponge=20, snapshot_codec=codec if mode in ("plain", "learned") else None,
-                           learned=mode == "learned", predictive=PREDICTIVE if mode == "predictive" else None)
+                           learned=mode == "learned", predictive=PREDICTIVE if mode.startswith("predictive") else None,
+                           residual_codec=rcodec if mode == "predictive_learned" else None)
         times[mode] = time.perf_counter() - t0
         if r["stored_bytes"]:
             stored[mode] = r["stored_bytes"]
         imgs[mode] = img.download(np.zeros(n ** 3, F32)).astype(np.float64)
         img.free()
     ref_n = np.linalg.norm(imgs["exact"])
-    gerr = {k: float(np.linalg.norm(imgs[k] - imgs["exact"]) / ref_n) for k in ("plain", "learned", "predictive")}
+    gerr = {k: float(np.linalg.norm(imgs[k] - imgs["exact"]) / ref_n)
+            for k in ("plain", "learned", "predictive", "predictive_learned")}
     for q in list(m1.values()) + list(m2.values()) + [dvb]:
         q.free()
     # rates on a big field
@@ -592,6 +609,7 @@ def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
     for q in (fb, cb, o, s1):
         q.free()
     codec.close()
+    rcodec.close()
     gb = 4 * big_n ** 3 / 1e9
     return {"ratio_512": round(lcodec.ratio((512, 512, 512)), 3), "ratio_window": round(lcodec.ratio(shape), 3),
             "train": {"snapshots": len(train), "grid": n, "iters": iters, "patch": 32, "batch": 4, "seconds":
@@ -602,7 +620,9 @@ def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
             "born_image_rel_err_vs_exact": {k: round(v, 6) for k, v in gerr.items()},
             "predictive": {"segment": PREDICTIVE[0], "order": PREDICTIVE[1], "keyframe_bits": PREDICTIVE[2],
                            "stored_ratio_vs_exact": round(4 * n ** 3 * nt / stored["predictive"], 3),
-                           "residuals": "11-level block quantiser"},
+                           "residuals": "11-level block quantiser; predictive_learned: decoded by a second "
+                                        "network trained on open-loop residual frames",
+                           "residual_train_loss_first_last": [float(rlosses[:20].mean()), float(rlosses[-50:].mean())]},
             "born_seconds": {k: round(v, 2) for k, v in times.items()},
             "rates_GBps_512": {"encode_learned": round(gb / t_enc, 1), "decode_learned": round(gb / t_dec, 1),
                                "encode_plus_dequantize_plain": round(gb / t_plain, 1)},

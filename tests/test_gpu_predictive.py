@@ -7,7 +7,8 @@
 * on smooth moving frames the residual coding is far more accurate than quantising the
   frames themselves at the same 3.59 bits per value;
 * a Born shot with the store gives an image within 1e-3 relative L2 of exact storage, with
-  8x fewer stored bytes."""
+  8x fewer stored bytes, also with its residuals decoded by a learned decoder trained on
+  residual frames of another model."""
 import numpy as np
 import pytest
 
@@ -66,14 +67,17 @@ def _frames(shape, nt, seed):
     return out
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
-def test_round_trip_bounds_and_reverse_decode(order):
+@pytest.mark.parametrize("order,learned", [(1, False), (2, False), (3, False), (3, True)])
+def test_round_trip_bounds_and_reverse_decode(order, learned):
+    # learned: residuals coded by a (seeded, untrained) learned decoder, whose correction stays
+    # inside the quantisation cell, so the same bounds hold
     from wavefd_amd import _lib, born, ginsu, lcodec
     shape = (40, 36, 32)  # (nz, ny, nx)
     n = int(np.prod(shape))
     nt, k = 37, 8
     frames = _frames(shape, nt, 1)
-    st = born._PredictiveStore(shape, nt, 1, k, order, 8)  # 8-bit keyframes: error <= max|x| / 254
+    net = lcodec.LearnedCodec(seed=3) if learned else None
+    st = born._PredictiveStore(shape, nt, 1, k, order, 8, codec=net)  # 8-bit keyframes: error <= max|x| / 254
     recon, bound = [], []
     hs = ginsu.DeviceBuffer(4 * n)
     for t in range(1, nt + 1):
@@ -113,9 +117,11 @@ def test_round_trip_bounds_and_reverse_decode(order):
     for x in (cb, q, hs):
         x.free()
     st.free()
+    if net is not None:
+        net.close()
 
 
-def _born_images(k, order, keybits, shape=(64, 56, 72), nt=240):
+def _born_images(k, order, keybits, shape=(64, 56, 72), nt=240, codec=None):
     """the Born image of one Iso shot with exact snapshots and with the predictive store;
     shape: the big grid (nx, ny, nz), whole-grid window"""
     from wavefd_amd import born, ginsu, spacetime
@@ -133,7 +139,8 @@ def _born_images(k, order, keybits, shape=(64, 56, 72), nt=240):
         img = ginsu.DeviceBuffer(dv.nbytes)
         img.zero()
         r = born.born_shot("iso", win, (10.0, 10.0, 10.0), 1e-3, sh, model, dvb.ptr, img.ptr, nsponge=10,
-                           predictive=(k, order, keybits) if mode == "predictive" else None)
+                           predictive=(k, order, keybits) if mode == "predictive" else None,
+                           residual_codec=codec if mode == "predictive" else None)
         imgs[mode] = img.download(np.zeros(shape, F32)).astype(np.float64)
         if mode == "predictive":
             stored = r["stored_bytes"]
@@ -160,3 +167,60 @@ def test_born_image_with_predictive_store_variants(k, order, keybits):
     # reported for comparison; bounded loosely
     rel, ratio = _born_images(k, order, keybits)
     assert ratio >= 7.0 and rel <= 5e-3
+
+
+def _residual_frames(model_seed, shape, nt, every=3):
+    """open-loop third-order temporal residuals DP_n - (3 DP_n-1 - 3 DP_n-2 + DP_n-3) of an Iso
+    shot (SaveDP steps), every `every` steps: what the predictive store codes"""
+    import wavefd_amd as w
+    from wavefd_amd import ginsu, lcodec, spacetime
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    p = w.Prop3DAcoIsoDenQ_DEO2_FDTD(nz=nz, ny=ny, nx=nx, dz=10.0, dy=10.0, dx=10.0, dt=1e-3, nsponge=10,
+                                     freesurface=True)
+    m = make_model("iso", shape, model_seed)
+    w.V(p)[...] = m["v"].T
+    w.B(p)[...] = m["b"].T
+    w.sync_to_device(p)
+    w.coherence(p, "device")
+    src = spacetime.hickscoeffs(F32(10), F32(10), F32(10), 0.0, 0.0, 0.0, nz, ny, nx, [45.0], [280.0], [320.0])
+    w.set_sources(p, [src], spacetime.causal_ricker(15.0, 1e-3 * np.arange(nt))[None, :])
+    ring = [ginsu.DeviceBuffer(4 * n) for _ in range(4)]
+    pred = ginsu.DeviceBuffer(4 * n)
+    strm = p._lib.GetStream(p.p)
+    out = []
+    for it in range(1, nt + 1):
+        assert p._lib.TimeStepSaveDP(p.p, ring[3].ptr) == 0
+        p._lib.InjectRegistered(p.p, it)
+        ring = [ring[3]] + ring[:3]
+        if it >= 4 and it % every == 0:
+            b = ginsu.DeviceBuffer(4 * n)
+            lcodec.combine(n, 3.0, ring[1].ptr, pred.ptr, -3.0, ring[2].ptr, 1.0, ring[3].ptr, stream=strm)
+            lcodec.combine(n, 1.0, ring[0].ptr, b.ptr, -1.0, pred.ptr, stream=strm)
+            out.append(b)
+    w.synchronize(p)
+    for q in ring + [pred]:
+        q.free()
+    w.free(p)
+    return out
+
+
+def test_learned_residual_decoder_on_another_model():
+    # a decoder trained on residual frames of one model, used on another: the same stored bytes,
+    # the image still within 1e-3 of exact storage.  Whether it lowers the error depends on the
+    # training frames (measured: 5.1e-4 -> 4.0e-4 in bench.py --codec; 1.25e-4 -> 1.68e-4 here),
+    # so the test bounds it rather than asserting a gain
+    from wavefd_amd import lcodec
+    shape = (64, 56, 72)
+    nx, ny, nz = shape
+    frames = _residual_frames(7, shape, 240)
+    net = lcodec.LearnedCodec(seed=11)
+    net.train([b.ptr for b in frames], (nz, ny, nx), iters=600, patch=32, batch=4, lr=1e-3, seed=11)
+    for b in frames:
+        b.free()
+    rel_plain, ratio = _born_images(48, 3, 16)
+    rel_learned, ratio_l = _born_images(48, 3, 16, codec=net)
+    net.close()
+    print(f"learned residual decoder: image rel err {rel_learned:.3e} against {rel_plain:.3e}", flush=True)
+    assert ratio_l == ratio
+    assert rel_plain <= 1e-3 and rel_learned <= 1e-3

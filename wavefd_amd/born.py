@@ -283,7 +283,7 @@ class _PredictiveStore:
 
     COEF = {1: (1.0,), 2: (2.0, -1.0), 3: (3.0, -3.0, 1.0)}
 
-    def __init__(self, shape, nt, nf, k=48, order=3, keybits=16):
+    def __init__(self, shape, nt, nf, k=48, order=3, keybits=16, codec=None):
         from . import lcodec, _lib
         if int(k) < 2 or int(order) not in self.COEF or int(keybits) not in (8, 16):
             raise ValueError("predictive store: segments of at least 2 frames, order 1, 2 or 3, 8- or 16-bit "
@@ -308,6 +308,26 @@ class _PredictiveStore:
         self.tmp = ginsu.DeviceBuffer(4 * ncell)
         self.seg = ginsu.DeviceBuffer(4 * ncell * min(self.k, nt) * nf)
         self.cached = None
+        # optional learned residual decoder (lcodec.LearnedCodec, trained on residual frames):
+        # its encode (with the per-block choice) and decode replace the plain quantiser's on both
+        # sides, so the reconstructions still agree bit for bit
+        self.codec = codec
+        self.scr = [ginsu.DeviceBuffer(4 * ncell) for _ in range(2)] if codec is not None else []
+
+    def _code_residual(self, code, stream):
+        """quantise the residual in tmp into code, then leave its decode in tmp"""
+        if self.codec is None:
+            self.lc.encode(self.tmp.ptr, self.shape, code, stream)
+            self.lc.dequantize(code, self.tmp.ptr, stream)
+        else:
+            self.codec.encode(self.tmp.ptr, self.shape, code, self.scr[0].ptr, self.scr[1].ptr, stream)
+            self.codec.decode(code, self.tmp.ptr, self.scr[0].ptr, stream)
+
+    def _decode_residual(self, code, stream):
+        if self.codec is None:
+            self.lc.dequantize(code, self.tmp.ptr, stream)
+        else:
+            self.codec.decode(code, self.tmp.ptr, self.scr[0].ptr, stream)
 
     def stored_bytes(self):
         return 4 * (self.kwords * self.nseg + self.words * (self.nt - self.nseg)) * self.nf
@@ -358,8 +378,7 @@ class _PredictiveStore:
             q = min(m, self.order)
             pred = self._predict([b.ptr for b in rec[:q]], self.pred[kind].ptr, stream)
             lc.combine(nc, 1.0, x, self.tmp.ptr, -1.0, pred, stream=stream)  # residual
-            lc.encode(self.tmp.ptr, self.shape, self._code(kind, n), stream)
-            lc.dequantize(self._code(kind, n), self.tmp.ptr, stream)
+            self._code_residual(self._code(kind, n), stream)
             # the reconstruction replaces the oldest one (no longer needed: the prediction is formed)
             lc.combine(nc, 1.0, pred, rec[-1].ptr, 1.0, self.tmp.ptr, stream=stream)
         self.rec[kind] = [rec[-1]] + rec[:-1]
@@ -372,7 +391,7 @@ class _PredictiveStore:
                 q = min(n - n0, self.order)
                 out = self._slot(kind, n)
                 pred = self._predict([self._slot(kind, n - i) for i in range(1, q + 1)], out, stream)
-                self.lc.dequantize(self._code(kind, n), self.tmp.ptr, stream)
+                self._decode_residual(self._code(kind, n), stream)
                 self.lc.combine(self.ncell, 1.0, pred, out, 1.0, self.tmp.ptr, stream=stream)
         self.cached = j
 
@@ -387,7 +406,8 @@ class _PredictiveStore:
         return self.rec[kind][0].ptr
 
     def free(self):
-        for b in [self.keys, self.codes, self.tmp, self.seg] + self.x + self.pred + [r for rs in self.rec for r in rs]:
+        for b in ([self.keys, self.codes, self.tmp, self.seg] + self.x + self.pred + self.scr +
+                  [r for rs in self.rec for r in rs]):
             b.free()
 
 
@@ -415,7 +435,8 @@ class _ShotRun:
     model, dv, the window image, the DP (DM) snapshot area and the shot's taps."""
 
     def __init__(self, physics, window, spacing, dt, shot, model, dmodel, nsponge, freesurface, freqQ, qMin,
-                 qInterior, snapshots, codec=None, learned=True, checkpoint=None, fused=True, predictive=None):
+                 qInterior, snapshots, codec=None, learned=True, checkpoint=None, fused=True, predictive=None,
+                 codec_for_residuals=None):
         if checkpoint is not None and (codec is not None or int(checkpoint) < 1):
             raise ValueError("checkpoint: an interval >= 1, and not together with a codec store")
         if predictive is not None and (codec is not None or checkpoint is not None or
@@ -453,7 +474,8 @@ class _ShotRun:
             self.store = _CheckpointStore(self, checkpoint, fused)
         elif predictive is not None:
             self.store = _PredictiveStore(window.size, nt, nf, *(predictive if isinstance(predictive, tuple)
-                                                                  else (predictive,)))
+                                                                  else (predictive,)),
+                                          codec=codec_for_residuals)
         elif codec is None:
             self.store = _ExactStore(ncell, nt, nf, snapshots)
         else:
@@ -650,7 +672,8 @@ def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
 
 def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=20, freesurface=True, freqQ=5.0,
               qMin=0.1, qInterior=100.0, snapshots=None, adjoint_data=None, keep_born_data=True, fused=True,
-              coupled=None, snapshot_codec=None, learned=True, checkpoint=None, predictive=None):
+              coupled=None, snapshot_codec=None, learned=True, checkpoint=None, predictive=None,
+              residual_codec=None):
     """Run the three passes of one shot on `window` and add its image into `image`.
 
     model: {'v','b'[,'eps','eta','f']} -> device address of the big grid's field; dmodel: device
@@ -668,7 +691,8 @@ def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=
     nt per wavefield).  predictive: a segment length k, or (k, order, keybits), to keep DP (DM) by
     closed-loop temporal prediction (``_PredictiveStore``: a 16-bit keyframe every k frames,
     quantised residuals of a third-order extrapolation in between; lossy, 8.2x fewer stored
-    bytes than float32 at k = 48).  Returns a dict with the Born
+    bytes than float32 at k = 48); residual_codec: a ``lcodec.LearnedCodec`` trained on residual
+    frames, to code them with the learned decoder.  Returns a dict with the Born
     data, the stored snapshot bytes and timings."""
     if physics not in _CLS:
         raise ValueError("physics must be 'iso' or 'vti'")
@@ -676,7 +700,7 @@ def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=
         coupled = fused and physics == "iso"
     t0 = time.perf_counter()
     run = _ShotRun(physics, window, spacing, dt, shot, model, dmodel, nsponge, freesurface, freqQ, qMin, qInterior,
-                   snapshots, snapshot_codec, learned, checkpoint, fused, predictive)
+                   snapshots, snapshot_codec, learned, checkpoint, fused, predictive, residual_codec)
     t1 = time.perf_counter()
     try:
         _run_passes([run], fused, None if adjoint_data is None else [adjoint_data], keep_born_data, coupled)
