@@ -359,7 +359,7 @@ def run_coherent(physics, n, steps, seed=3):
 
 
 def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000), fused=True, batch=1,
-             snapshot_gb=180.0, checkpoint=None):
+             snapshot_gb=180.0, checkpoint=None, predictive=None):
     """Born shots/hour: `nshots` shots on seeded ginsu windows (per-axis sizes U{200..600}, from the
     surface down, horizontal origin uniform in the grid) of a big (nz, ny, nx) model at 12.5 m,
     dt 1 ms, nt steps each: background forward saving DP (and DM), Born forward with a seeded
@@ -369,7 +369,9 @@ def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000), fused=Tr
     False: the separate calls of the reference's API.  batch > 1: up to `batch` shots (and at most
     snapshot_gb of DP snapshots) run together, each time step of all of them one launch;
     batch 1: shots one after another.  checkpoint: recompute DP (DM) from background checkpoints
-    every `checkpoint` steps instead of storing all nt (the same image bit for bit)."""
+    every `checkpoint` steps instead of storing all nt (the same image bit for bit); predictive:
+    keep them in the predictive store with segments of that many frames (lossy, about 8x fewer
+    bytes)."""
     import torch
     from wavefd_amd import born, ginsu
 
@@ -409,7 +411,8 @@ def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000), fused=Tr
     setup_s = time.time() - t0
     nf = 1 if physics == "iso" else 2
     maxcells = max(w.ncells for w in windows)
-    snap = ginsu.DeviceBuffer(4 * nt * nf * maxcells) if batch <= 1 and checkpoint is None else None
+    snap = (ginsu.DeviceBuffer(4 * nt * nf * maxcells) if batch <= 1 and checkpoint is None and predictive is None
+            else None)
     wav = np.asarray(born.spacetime.causal_ricker(15.0, dt * np.arange(nt)), np.float32)
     shots = []
     for w in windows:
@@ -422,11 +425,12 @@ def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000), fused=Tr
     t1 = time.perf_counter()
     if batch > 1:
         born.born_shots(physics, windows, (d, d, d), dt, shots, fields, dvb.ptr, image.ptr, nsponge=30, batch=batch,
-                        snapshot_bytes=snapshot_gb * 1e9, keep_born_data=False, fused=fused, checkpoint=checkpoint)
+                        snapshot_bytes=snapshot_gb * 1e9, keep_born_data=False, fused=fused, checkpoint=checkpoint,
+                        predictive=predictive)
     else:
         for w, sh in zip(windows, shots):
             born.born_shot(physics, w, (d, d, d), dt, sh, fields, dvb.ptr, image.ptr, nsponge=30, snapshots=snap,
-                           keep_born_data=False, fused=fused, checkpoint=checkpoint)
+                           keep_born_data=False, fused=fused, checkpoint=checkpoint, predictive=predictive)
     image.sync()
     wall = time.perf_counter() - t1
     img = image.download(np.zeros((nx, ny, nz), np.float32))
@@ -442,9 +446,12 @@ def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000), fused=Tr
             "mcell_steps_per_s": round(3 * nt * cells / wall / 1e6, 1), "finite": finite,
             "setup_s": round(setup_s, 2), "window_cells_total": cells,
             "passes": "forward (DP snapshots) + Born forward + adjoint imaging",
-            "snapshot_store": "exact" if checkpoint is None else f"checkpoints every {checkpoint} steps, recomputed",
-            "snapshot_gb_largest_window": round(4e-9 * maxcells * (nt * nf if checkpoint is None else
-                                                                   born.checkpoint_fields(nt, nf, checkpoint)), 2),
+            "snapshot_store": ("exact" if checkpoint is None and predictive is None else
+                               f"checkpoints every {checkpoint} steps, recomputed" if checkpoint is not None else
+                               f"predictive, segments of {predictive} frames (lossy)"),
+            "snapshot_gb_largest_window": round(4e-9 * maxcells * (
+                born.checkpoint_fields(nt, nf, checkpoint) if checkpoint is not None else
+                nt * nf / 8.2 + min(predictive, nt) * nf if predictive is not None else nt * nf), 2),
             "snapshot_gb_largest_window_exact": round(4e-9 * maxcells * nt * nf, 2),
             "shots_at_once": batch if batch > 1 else 1,
             "steps": ("one launch per time step of a batch" if batch > 1 else "fused pairs") if fused
@@ -714,6 +721,8 @@ def main():
     ap.add_argument("--born-nt", type=int, default=200)
     ap.add_argument("--born-plain", action="store_true", help="Born with the separate per-step calls")
     ap.add_argument("--born-batch", type=int, default=1, help="Born shots run together (one launch per step)")
+    ap.add_argument("--born-predictive", type=int, default=None,
+                    help="Born: keep DP in the predictive store with segments of K frames (lossy, ~8x fewer bytes)")
     ap.add_argument("--born-checkpoint", type=int, default=None,
                     help="Born: recompute DP from background checkpoints every K steps instead of storing them")
     ap.add_argument("--codec", action="store_true", help="only train and measure the learned snapshot codec")
@@ -745,7 +754,8 @@ def main():
         return
     if args.born:
         print(json.dumps({"born": run_born(args.born, args.born_shots, args.born_nt, fused=not args.born_plain,
-                                           batch=args.born_batch, checkpoint=args.born_checkpoint)}), flush=True)
+                                           batch=args.born_batch, checkpoint=args.born_checkpoint,
+                                           predictive=args.born_predictive)}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the propagator state of one shot lives on one MI355X and this "

@@ -715,14 +715,15 @@ def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=
 
 def born_shots(physics, windows, spacing, dt, shots, model, dmodel, image, nsponge=20, freesurface=True, freqQ=5.0,
                qMin=0.1, qInterior=100.0, batch=4, snapshot_bytes=None, adjoint_data=None, keep_born_data=True,
-               fused=True, checkpoint=None):
+               fused=True, checkpoint=None, predictive=None):
     """Born modelling and imaging of many shots (one window each, equal nt), several at a time:
     the shots of a batch run their passes together, each time step of all of them one launch
     of the fused kernel (TimeStepBatch).  Batches are formed in order, at most `batch` shots and,
     if snapshot_bytes is given, at most that many bytes of DP (DM) snapshots (a shot larger than
     that runs alone).  Every shot's results equal born_shot's for it; the images are added into
-    `image` one shot after the other, in shot order.  checkpoint: as in born_shot (the bytes a
-    shot counts against snapshot_bytes are then its ``checkpoint_fields``).  Returns the list of
+    `image` one shot after the other, in shot order.  checkpoint, predictive: as in born_shot (the
+    bytes a shot counts against snapshot_bytes are then its ``checkpoint_fields``, or its
+    predictive store's stored bytes and buffers).  Returns the list of
     per-shot Born data (None entries unless keep_born_data)."""
     if physics not in _CLS:
         raise ValueError("physics must be 'iso' or 'vti'")
@@ -736,7 +737,13 @@ def born_shots(physics, windows, spacing, dt, shots, model, dmodel, image, nspon
     batch = max(1, min(int(batch), 8))
     groups, cur, cur_bytes = [], [], 0
     for i, w in enumerate(windows):
-        b = 4 * w.ncells * (nt * nf if checkpoint is None else checkpoint_fields(nt, nf, checkpoint))
+        if checkpoint is not None:
+            b = 4 * w.ncells * checkpoint_fields(nt, nf, checkpoint)
+        elif predictive is not None:
+            k = predictive[0] if isinstance(predictive, tuple) else predictive
+            b = 4 * w.ncells * (nt * nf / 8.0 + (min(k, nt) + 6) * nf)  # codes, decode and encoder buffers
+        else:
+            b = 4 * w.ncells * nt * nf
         if cur and (len(cur) == batch or (snapshot_bytes is not None and cur_bytes + b > snapshot_bytes)):
             groups.append(cur)
             cur, cur_bytes = [], 0
@@ -750,7 +757,8 @@ def born_shots(physics, windows, spacing, dt, shots, model, dmodel, image, nspon
         try:
             for i in g:
                 runs.append(_ShotRun(physics, windows[i], spacing, dt, shots[i], model, dmodel, nsponge, freesurface,
-                                     freqQ, qMin, qInterior, None, checkpoint=checkpoint, fused=fused))
+                                     freqQ, qMin, qInterior, None, checkpoint=checkpoint, fused=fused,
+                                     predictive=predictive))
             _run_passes(runs, fused, None if adjoint_data is None else [adjoint_data[i] for i in g], keep_born_data,
                         coupled=fused and physics == "iso" and len(runs) == 1)
             for i, r in zip(g, runs):
