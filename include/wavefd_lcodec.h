@@ -31,6 +31,8 @@ extern "C" {
 long wavefd_lcodec_code_words(long n1, long n2, long n3);  /* code size in 32-bit words */
 long wavefd_lcodec_encode(const float *x, long n1, long n2, long n3, uint32_t *code, void *stream);
 long wavefd_lcodec_dequantize(const uint32_t *code, float *out, void *stream);  /* plain decode */
+/* the plain decode of a code of an n1 x n2 x n3 field, without reading its header (no host wait) */
+long wavefd_lcodec_dequantize_n(const uint32_t *code, long n1, long n2, long n3, float *out, void *stream);
 float wavefd_lcodec_norm(const uint32_t *code);  /* the code's largest block scale; -1 on error */
 /* half a quantisation step of every value's block, s / 10: |x - x_hat| is at most that */
 long wavefd_lcodec_halfstep(const uint32_t *code, float *out, void *stream);

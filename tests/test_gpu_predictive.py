@@ -224,3 +224,19 @@ def test_learned_residual_decoder_on_another_model():
     print(f"learned residual decoder: image rel err {rel_learned:.3e} against {rel_plain:.3e}", flush=True)
     assert ratio_l == ratio
     assert rel_plain <= 1e-3 and rel_learned <= 1e-3
+
+
+def test_dequantize_with_known_shape_equals_header_read():
+    from wavefd_amd import ginsu, lcodec
+    shape = (40, 36, 30)
+    n = int(np.prod(shape))
+    x = smooth(np.random.default_rng(9), shape, 3.0, 2).ravel()
+    xb = _dev(x)
+    cb = ginsu.DeviceBuffer(4 * lcodec.code_words(shape))
+    a, b = ginsu.DeviceBuffer(4 * n), ginsu.DeviceBuffer(4 * n)
+    lcodec.encode(xb.ptr, shape, cb.ptr)
+    lcodec.dequantize(cb.ptr, a.ptr)
+    lcodec.dequantize(cb.ptr, b.ptr, shape=shape)
+    assert np.array_equal(_get(a.ptr, n), _get(b.ptr, n))
+    for q in (xb, cb, a, b):
+        q.free()

@@ -134,6 +134,7 @@ LCODEC = {
     "wavefd_lcodec_code_words": (_L, [_L, _L, _L]),
     "wavefd_lcodec_encode": (_L, [_P, _L, _L, _L, _P, _P]),
     "wavefd_lcodec_dequantize": (_L, [_P, _P, _P]),
+    "wavefd_lcodec_dequantize_n": (_L, [_P, _L, _L, _L, _P, _P]),
     "wavefd_lcodec_nparams": (_L, []),
     "wavefd_lcodec_net_create": (_P, []),
     "wavefd_lcodec_net_free": (None, [_P]),

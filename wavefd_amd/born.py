@@ -318,14 +318,14 @@ class _PredictiveStore:
         """quantise the residual in tmp into code, then leave its decode in tmp"""
         if self.codec is None:
             self.lc.encode(self.tmp.ptr, self.shape, code, stream)
-            self.lc.dequantize(code, self.tmp.ptr, stream)
+            self.lc.dequantize(code, self.tmp.ptr, stream, self.shape)
         else:
             self.codec.encode(self.tmp.ptr, self.shape, code, self.scr[0].ptr, self.scr[1].ptr, stream)
             self.codec.decode(code, self.tmp.ptr, self.scr[0].ptr, stream)
 
     def _decode_residual(self, code, stream):
         if self.codec is None:
-            self.lc.dequantize(code, self.tmp.ptr, stream)
+            self.lc.dequantize(code, self.tmp.ptr, stream, self.shape)
         else:
             self.codec.decode(code, self.tmp.ptr, self.scr[0].ptr, stream)
 

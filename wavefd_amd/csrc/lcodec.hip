@@ -108,46 +108,58 @@ __device__ __forceinline__ long block_elem(const QGeo &g, long blk, int e, bool 
     return in ? (i3 * g.n2 + i2) * g.n1 + i1 : 0;
 }
 
-__global__ void __launch_bounds__(256) k_encode(QGeo g, const float *__restrict__ x, uint32_t *__restrict__ code) {
+// One wavefront per 8^3 block: lane t owns the block's row t (i2 = 8 k2 + t % 8, i3 = 8 k3 + t / 8),
+// i.e. its elements e = 8 t .. 8 t + 7 (e = (i1 & 7) + 8 (i2 & 7) + 64 (i3 & 7)); the block's
+// coordinates are formed once, the maximum is a wave reduction, and the base-11 digits pass
+// through LDS between the row layout and the 57 payload words (9 digits each, e = 9 w + j).
+__device__ __forceinline__ long row_base(const QGeo &g, long blk, int t, long &k1, bool &row_in) {
+    k1 = blk % g.m1;
+    const long rest = blk / g.m1, k2 = rest % g.m2, k3 = rest / g.m2;
+    const long i2 = k2 * BS + (t & 7), i3 = k3 * BS + (t >> 3);
+    row_in = i2 < g.n2 && i3 < g.n3;
+    return row_in ? (i3 * g.n2 + i2) * g.n1 + k1 * BS : 0;
+}
+
+__global__ void __launch_bounds__(64) k_encode(QGeo g, const float *__restrict__ x, uint32_t *__restrict__ code) {
     const long blk = blockIdx.x;
-    __shared__ float red[256];
-    __shared__ float sv[BV];
+    const int t = threadIdx.x;
+    __shared__ uint8_t dig[BV];
+    long k1;
+    bool row_in;
+    const long base = row_base(g, blk, t, k1, row_in);
+    const long nrow = g.n1 - k1 * BS;  // elements of the row inside the grid
+    float v[BS];
     float m = 0.0f;
-    for (int e = threadIdx.x; e < BV; e += 256) {
-        bool in;
-        const long k = block_elem(g, blk, e, in);
-        const float v = in ? x[k] : 0.0f;
-        sv[e] = v;
-        m = fmaxf(m, fabsf(v));
+#pragma unroll
+    for (int j = 0; j < BS; j++) {
+        v[j] = (row_in && j < nrow) ? x[base + j] : 0.0f;
+        m = fmaxf(m, fabsf(v[j]));
     }
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    const uint16_t sb = f2bf_up(red[0]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    const uint16_t sb = f2bf_up(m);
     const float s = bf2f(sb);
-    uint16_t *scales = reinterpret_cast<uint16_t *>(code + 8);
-    if (threadIdx.x == 0) {
-        scales[blk] = sb;
-        // the largest block scale (non-negative floats order as their bit patterns)
-        atomicMax(code + 7, (uint32_t)sb << 16);
+#pragma unroll
+    for (int j = 0; j < BS; j++) {
+        int q = 0;
+        if (s > 0.0f) {
+            q = (int)rintf(v[j] / s * (float)LV);
+            q = q < -LV ? -LV : (q > LV ? LV : q);
+        }
+        dig[t * BS + j] = (uint8_t)(q + LV);
     }
+    uint16_t *scales = reinterpret_cast<uint16_t *>(code + 8);
+    if (t == 0) scales[blk] = sb;  // the largest of them goes to header word 7 (k_norm)
+    __syncthreads();
     uint32_t *pay = code + pay_off(g) + WPB * blk;
-    if (threadIdx.x < WPB) {
+    if (t < WPB) {
         uint32_t w = 0, p = 1;
         for (int j = 0; j < 9; j++) {
-            const int e = threadIdx.x * 9 + j;
-            int q = 0;
-            if (e < BV && s > 0.0f) {
-                q = (int)rintf(sv[e] / s * (float)LV);
-                q = q < -LV ? -LV : (q > LV ? LV : q);
-            }
-            w += (uint32_t)(q + LV) * p;
+            const int e = t * 9 + j;
+            w += (e < BV ? (uint32_t)dig[e] : (uint32_t)LV) * p;
             p *= 11u;
         }
-        pay[threadIdx.x] = w;
+        pay[t] = w;
     }
 }
 
@@ -165,24 +177,46 @@ __global__ void __launch_bounds__(256) k_halfstep(QGeo g, const uint32_t *__rest
     }
 }
 
-__global__ void __launch_bounds__(256) k_dequantize(QGeo g, const uint32_t *__restrict__ code, float *__restrict__ out) {
+// header word 7: the largest block scale, as the bit pattern of a float (non-negative bfloat16
+// scales order as their bits); one workgroup, after k_encode (a single atomic word updated by
+// every block serialises them)
+__global__ void __launch_bounds__(1024) k_norm(QGeo g, uint32_t *__restrict__ code) {
+    __shared__ uint32_t red[1024];
+    const uint16_t *scales = reinterpret_cast<const uint16_t *>(code + 8);
+    uint32_t m = 0;
+    for (long b = threadIdx.x; b < g.nblocks(); b += 1024) m = max(m, (uint32_t)scales[b]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = 512; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) code[7] = red[0] << 16;
+}
+
+__global__ void __launch_bounds__(64) k_dequantize(QGeo g, const uint32_t *__restrict__ code, float *__restrict__ out) {
     const long blk = blockIdx.x;
+    const int t = threadIdx.x;
+    __shared__ uint8_t dig[9 * WPB];
     const uint16_t *scales = reinterpret_cast<const uint16_t *>(code + 8);
     const float s = bf2f(scales[blk]);
     const uint32_t *pay = code + pay_off(g) + WPB * blk;
-    if (threadIdx.x < WPB) {
-        uint32_t w = pay[threadIdx.x];
+    if (t < WPB) {
+        uint32_t w = pay[t];
         for (int j = 0; j < 9; j++) {
-            const int e = threadIdx.x * 9 + j;
-            const int d = (int)(w % 11u);
+            dig[t * 9 + j] = (uint8_t)(w % 11u);
             w /= 11u;
-            if (e < BV) {
-                bool in;
-                const long k = block_elem(g, blk, e, in);
-                if (in) out[k] = dequant(s, d);
-            }
         }
     }
+    __syncthreads();
+    long k1;
+    bool row_in;
+    const long base = row_base(g, blk, t, k1, row_in);
+    const long nrow = g.n1 - k1 * BS;
+    if (!row_in) return;
+#pragma unroll
+    for (int j = 0; j < BS; j++)
+        if (j < nrow) out[base + j] = dequant(s, dig[t * BS + j]);
 }
 
 // encoder's choice per block: flag the block when the learned decode o is closer to the exact x
@@ -601,7 +635,7 @@ void host_mask(std::vector<float> &mk) {
 // out = ca a + cb b + cc c, b and c optional, evaluated in that order: the combinations of a
 // closed-loop temporal predictor (prediction 2 a - b, residual x - prediction, reconstruction
 // prediction + decoded residual), the same kernel on the encoding and the decoding side
-__global__ void k_combine(long n, float ca, const float *__restrict__ a, float cb, const float *__restrict__ b,
+__global__ void k_lincomb(long n, float ca, const float *__restrict__ a, float cb, const float *__restrict__ b,
                           float cc, const float *__restrict__ c, float *__restrict__ out) {
     const long stride = (long)gridDim.x * blockDim.x;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -610,6 +644,14 @@ __global__ void k_combine(long n, float ca, const float *__restrict__ a, float c
         if (c) v = v + cc * c[i];
         out[i] = v;
     }
+}
+
+// the code's header (magic, grid, block size, levels, digits per word; word 7, the largest
+// block scale, is written by k_norm after the blocks), written on the stream so that encoding
+// needs no host memory and no wait
+__global__ void k_header(uint32_t *code, uint32_t n1, uint32_t n2, uint32_t n3) {
+    const uint32_t h[8] = {MAGIC, n1, n2, n3, (uint32_t)BS, (uint32_t)LV, 9u, 0u};
+    if (threadIdx.x < 8) code[threadIdx.x] = h[threadIdx.x];
 }
 
 }  // namespace
@@ -627,7 +669,7 @@ long wavefd_lcodec_combine(long n, float ca, const float *a, float cb, const flo
     }
     if (n == 0) return 0;
     const long blocks = std::min<long>((n + 255) / 256, 1L << 20);
-    k_combine<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(n, ca, a, cb, b, cc, c, out);
+    k_lincomb<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(n, ca, a, cb, b, cc, c, out);
     LC_TRY(hipGetLastError());
     return 0;
 }
@@ -644,12 +686,13 @@ long wavefd_lcodec_encode(const float *x, long n1, long n2, long n3, uint32_t *c
     }
     const QGeo g = qgeo(n1, n2, n3);
     hipStream_t s = (hipStream_t)stream;
-    const uint32_t hdr[8] = {MAGIC, (uint32_t)n1, (uint32_t)n2, (uint32_t)n3, (uint32_t)BS, (uint32_t)LV, 9u, 0u};
-    LC_TRY(hipMemcpyAsync(code, hdr, sizeof(hdr), hipMemcpyHostToDevice, s));
-    LC_TRY(hipMemsetAsync(code + 8, 0, sizeof(uint32_t) * (pay_off(g) - 8), s));  // scales and flags
-    k_encode<<<(unsigned)g.nblocks(), 256, 0, s>>>(g, x, code);
+    k_header<<<1, 64, 0, s>>>(code, (uint32_t)n1, (uint32_t)n2, (uint32_t)n3);
     LC_TRY(hipGetLastError());
-    LC_TRY(hipStreamSynchronize(s));  // the header copy reads host memory
+    LC_TRY(hipMemsetAsync(code + 8, 0, sizeof(uint32_t) * (pay_off(g) - 8), s));  // scales and flags
+    k_encode<<<(unsigned)g.nblocks(), 64, 0, s>>>(g, x, code);
+    LC_TRY(hipGetLastError());
+    k_norm<<<1, 1024, 0, s>>>(g, code);
+    LC_TRY(hipGetLastError());
     return 0;
 }
 
@@ -687,6 +730,18 @@ long wavefd_lcodec_dequantize(const uint32_t *code, float *out, void *stream) {
     if (read_header(code, n, s)) return -1;
     const QGeo g = qgeo(n[0], n[1], n[2]);
     k_dequantize<<<(unsigned)g.nblocks(), 64, 0, s>>>(g, code, out);
+    LC_TRY(hipGetLastError());
+    return 0;
+}
+
+// the plain decode of a code whose grid the caller knows (no header read, no host wait)
+long wavefd_lcodec_dequantize_n(const uint32_t *code, long n1, long n2, long n3, float *out, void *stream) {
+    if (n1 <= 0 || n2 <= 0 || n3 <= 0 || !code || !out) {
+        g_err = "dequantize_n: bad arguments";
+        return -1;
+    }
+    const QGeo g = qgeo(n1, n2, n3);
+    k_dequantize<<<(unsigned)g.nblocks(), 64, 0, (hipStream_t)stream>>>(g, code, out);
     LC_TRY(hipGetLastError());
     return 0;
 }

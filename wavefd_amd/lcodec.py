@@ -39,9 +39,15 @@ def encode(field, shape, code, stream=None):
         raise RuntimeError(_err())
 
 
-def dequantize(code, out, stream=None):
-    """the plain decode (no network): out = s (d - 5) / 5"""
-    if _lib.load("lcodec").wavefd_lcodec_dequantize(int(code), int(out), stream) != 0:
+def dequantize(code, out, stream=None, shape=None):
+    """the plain decode (no network): out = s (d - 5) / 5; with the field's (nz, ny, nx) shape
+    given, without reading the code's header (no host wait)"""
+    lib = _lib.load("lcodec")
+    if shape is None:
+        rc = lib.wavefd_lcodec_dequantize(int(code), int(out), stream)
+    else:
+        rc = lib.wavefd_lcodec_dequantize_n(int(code), *[int(v) for v in shape], int(out), stream)
+    if rc != 0:
         raise RuntimeError(_err())
 
 
