@@ -41,6 +41,10 @@ long wavefd_lcodec_halfstep(const uint32_t *code, float *out, void *stream);
  * the same call on the encoding and the decoding side; no counterpart in the reference */
 long wavefd_lcodec_combine(long n, float ca, const float *a, float cb, const float *b, float cc, const float *c,
                            float *out, void *stream);
+/* the same store's encoder pass (the values the combine calls above would give): pred = ca a + cb b
+ * + cc c and res = x - pred in one pass */
+long wavefd_lcodec_predict_residual(long n, float ca, const float *a, float cb, const float *b, float cc,
+                                    const float *c, const float *x, float *pred, float *res, void *stream);
 
 /* the decoder network */
 long wavefd_lcodec_nparams(void);

@@ -150,6 +150,7 @@ LCODEC = {
     "wavefd_lcodec_gather": (_L, [_P, _L, _L, _L, _P, _L, _L, ctypes.c_float, _P, _P]),
     "wavefd_lcodec_norm": (ctypes.c_float, [_P]),
     "wavefd_lcodec_combine": (_L, [_L, _F, _P, _F, _P, _F, _P, _P, _P]),
+    "wavefd_lcodec_predict_residual": (_L, [_L, _F, _P, _F, _P, _F, _P, _P, _P, _P, _P]),
     "wavefd_lcodec_LastError": (ctypes.c_char_p, []),
 }
 

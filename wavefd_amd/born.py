@@ -374,12 +374,20 @@ class _PredictiveStore:
             if self.cod.wavefd_codec_compress(x, self.n3, self.b3, self.keybits, self._key(kind, (n - 1) // self.k)) < 0:
                 raise RuntimeError(_lib_error("codec"))
             self._keyframe_decode(kind, (n - 1) // self.k, rec[-1].ptr, stream)
+        elif self.codec is None:
+            # prediction and residual in one pass; the reconstruction (prediction + decoded residual)
+            # replaces the oldest one (no longer needed: the prediction is formed)
+            q = min(m, self.order)
+            pred = self.pred[kind].ptr
+            lc.predict_residual(nc, self.COEF[q], [b.ptr for b in rec[:q]], x, pred, self.tmp.ptr, stream)
+            lc.encode(self.tmp.ptr, self.shape, self._code(kind, n), stream)
+            lc.dequantize(self._code(kind, n), self.tmp.ptr, stream, self.shape)
+            lc.combine(nc, 1.0, pred, rec[-1].ptr, 1.0, self.tmp.ptr, stream=stream)
         else:
             q = min(m, self.order)
             pred = self._predict([b.ptr for b in rec[:q]], self.pred[kind].ptr, stream)
             lc.combine(nc, 1.0, x, self.tmp.ptr, -1.0, pred, stream=stream)  # residual
             self._code_residual(self._code(kind, n), stream)
-            # the reconstruction replaces the oldest one (no longer needed: the prediction is formed)
             lc.combine(nc, 1.0, pred, rec[-1].ptr, 1.0, self.tmp.ptr, stream=stream)
         self.rec[kind] = [rec[-1]] + rec[:-1]
 
@@ -390,7 +398,8 @@ class _PredictiveStore:
             for n in range(n0 + 1, min(n0 + self.k, self.nt + 1)):
                 q = min(n - n0, self.order)
                 out = self._slot(kind, n)
-                pred = self._predict([self._slot(kind, n - i) for i in range(1, q + 1)], out, stream)
+                prev = [self._slot(kind, n - i) for i in range(1, q + 1)]
+                pred = self._predict(prev, out, stream)
                 self._decode_residual(self._code(kind, n), stream)
                 self.lc.combine(self.ncell, 1.0, pred, out, 1.0, self.tmp.ptr, stream=stream)
         self.cached = j

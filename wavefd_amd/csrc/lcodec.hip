@@ -177,6 +177,21 @@ __global__ void __launch_bounds__(256) k_halfstep(QGeo g, const uint32_t *__rest
     }
 }
 
+// the predictive store's encoder: pred = ca a + cb b + cc c (b, c optional, in that order) and
+// res = x - pred in one pass (the values k_lincomb would give for each)
+__global__ void k_pred_res(long n, float ca, const float *__restrict__ a, float cb, const float *__restrict__ b,
+                           float cc, const float *__restrict__ c, const float *__restrict__ x,
+                           float *__restrict__ pred, float *__restrict__ res) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float v = ca * a[i];
+        if (b) v = v + cb * b[i];
+        if (c) v = v + cc * c[i];
+        pred[i] = v;
+        res[i] = x[i] + -1.0f * v;
+    }
+}
+
 // header word 7: the largest block scale, as the bit pattern of a float (non-negative bfloat16
 // scales order as their bits); one workgroup, after k_encode (a single atomic word updated by
 // every block serialises them)
@@ -742,6 +757,20 @@ long wavefd_lcodec_dequantize_n(const uint32_t *code, long n1, long n2, long n3,
     }
     const QGeo g = qgeo(n1, n2, n3);
     k_dequantize<<<(unsigned)g.nblocks(), 64, 0, (hipStream_t)stream>>>(g, code, out);
+    LC_TRY(hipGetLastError());
+    return 0;
+}
+
+// pred = ca a + cb b + cc c and res = x - pred over n floats (device; b, c may be NULL)
+long wavefd_lcodec_predict_residual(long n, float ca, const float *a, float cb, const float *b, float cc,
+                                    const float *c, const float *x, float *pred, float *res, void *stream) {
+    if (n < 0 || (n > 0 && (!a || !x || !pred || !res))) {
+        g_err = "predict_residual: bad arguments";
+        return -1;
+    }
+    if (n == 0) return 0;
+    const long blocks = std::min<long>((n + 255) / 256, 1L << 20);
+    k_pred_res<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(n, ca, a, cb, b, cc, c, x, pred, res);
     LC_TRY(hipGetLastError());
     return 0;
 }

@@ -58,6 +58,20 @@ def combine(n, ca, a, out, cb=0.0, b=None, cc=0.0, c=None, stream=None):
         raise RuntimeError(_err())
 
 
+def _opt(p):
+    return None if p is None else int(p)
+
+
+def predict_residual(n, coef, prev, x, pred, res, stream=None):
+    """pred = sum coef[i] prev[i] (up to 3 terms, in order) and res = x - pred, in one pass"""
+    c = list(coef) + [0.0] * (3 - len(coef))
+    p = list(prev) + [None] * (3 - len(prev))
+    if _lib.load("lcodec").wavefd_lcodec_predict_residual(int(n), float(c[0]), int(p[0]), float(c[1]), _opt(p[1]),
+                                                          float(c[2]), _opt(p[2]), int(x), int(pred), int(res),
+                                                          stream) != 0:
+        raise RuntimeError(_err())
+
+
 class LearnedCodec:
     """The quantiser with its learned decoder.  Parameters: per layer W[t][co][c] (27 x 16 x 16,
     t = (dx+1) 9 + (dy+1) 3 + (dz+1)) and 16 biases, three layers (conv1 reads channel 0 only,
