@@ -1,4 +1,4 @@
-"""Host logic of the Born driver's checkpointed snapshot store (no GPU): the segment slots of
+"""Host logic of the Born driver's checkpointed and predictive snapshot stores (no GPU): the segment slots of
 every step and wavefield are distinct and inside the segment buffer, checkpoints are taken
 before steps 1, k+1, 2k+1, ..., the coupled forward pass writes the last segment straight into
 the segment buffer, a separate Born pass runs the background segment by segment with no
@@ -82,3 +82,31 @@ def test_memory_formula():
     assert born.checkpoint_fields(200, 1, 20) == 2 * 10 + 20 + 6
     assert born.checkpoint_fields(3000, 2, 77) == 2 * 2 * 39 + 77 * 2 + 12
     assert born.checkpoint_fields(10, 1, 50) == 2 + 10 + 6
+
+
+@pytest.mark.parametrize("nt,k", [(37, 8), (48, 48), (200, 48), (10, 16), (1, 4)])
+@pytest.mark.parametrize("nf", [1, 2])
+def test_predictive_store_layout(monkeypatch, nt, k, nf):
+    # every residual code, keyframe and segment slot of the predictive store is distinct and
+    # inside its buffer; keyframes are the first frame of each segment
+    monkeypatch.setattr(born.ginsu, "DeviceBuffer", _Buf)
+    shape = (20, 18, 17)
+    st = born._PredictiveStore(shape, nt, nf, k, 3, 16)
+    codes, keys, slots = set(), set(), set()
+    for n in range(1, nt + 1):
+        for kind in range(nf):
+            if (n - 1) % k == 0:
+                keys.add(st._key(kind, (n - 1) // k))
+            else:
+                c = st._code(kind, n)
+                assert st.codes.ptr <= c and c + 4 * st.words <= st.codes.ptr + st.codes.nbytes
+                codes.add(c)
+            if (n - 1) // k == 0:
+                s = st._slot(kind, n)
+                assert st.seg.ptr <= s and s + 4 * st.ncell <= st.seg.ptr + st.seg.nbytes
+                slots.add(s)
+    nseg = (nt + k - 1) // k
+    assert len(keys) == nseg * nf and len(codes) == (nt - nseg) * nf
+    assert len(slots) == min(k, nt) * nf
+    assert max(keys) + 4 * st.kwords <= st.keys.ptr + st.keys.nbytes
+    assert st.stored_bytes() == 4 * (st.kwords * nseg + st.words * (nt - nseg)) * nf
