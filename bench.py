@@ -633,7 +633,7 @@ def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
             "born_seconds": {k: round(v, 2) for k, v in times.items()},
             "rates_GBps_512": {"encode_learned": round(gb / t_enc, 1), "decode_learned": round(gb / t_dec, 1),
                                "encode_plus_dequantize_plain": round(gb / t_plain, 1)},
-            "target": "north star: >= 8x and Born gradient rel L2 <= 1e-3 against exact storage"}
+            "target": ">= 8x fewer snapshot bytes with the Born gradient within rel L2 1e-3 of exact storage"}
 
 
 def cpu_baseline(physics, n, target_s=12.0):

@@ -1,8 +1,8 @@
 """CPU restatement of the learned snapshot codec -- TEST INFRASTRUCTURE ONLY.
 
 The product (``wavefd_amd/csrc/lcodec.hip``) has no counterpart in the reference (whose
-snapshot store is CvxCompress or exact storage, src/compressedio.jl:1-148); this module is the
-checker for it:
+snapshot store is CvxCompress or exact storage, src/compressedio.jl:1-148); this module is what
+its kernels are compared against:
 
 * ``quantize`` / ``dequantize``: 8^3 blocks, scale = block max |x| rounded up to bfloat16,
   q = rint(5 x / s) clipped to -5..5, x_hat = s (d - 5) / 5 in float32 -- bit-exact with the GPU;

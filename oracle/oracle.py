@@ -1,7 +1,7 @@
 """CPU oracle for the WaveFD 3D acoustic path -- TEST INFRASTRUCTURE ONLY.
 
-This module is the checker: tests, ``__graft_entry__.smoke()`` and the ``cpu_baseline``
-leg of ``bench.py`` compare the HIP product path against it.  The product package
+Tests, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` compare the
+HIP product path against it.  The product package
 (``wavefd_amd``) never imports it.
 
 It wraps ``oracle/lib/libwavefd_oracle.so`` (a C restatement of the reference kernels,

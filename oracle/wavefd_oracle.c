@@ -1,8 +1,8 @@
 /*
  * wavefd_oracle.c -- CPU restatement of WaveFD's 3D acoustic DenQ DEO2 FDTD path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP product path is compared
- * against, and the "port" CPU baseline that bench.py times.  Nothing in the product
+ * TEST INFRASTRUCTURE ONLY.  This file is what the HIP product path is compared against,
+ * and the "port" CPU baseline that bench.py times.  Nothing in the product
  * (wavefd_amd/, include/) includes, links or calls it.
  *
  * It restates, in plain C99 + OpenMP, the reference C++ kernels listed per function

@@ -295,7 +295,8 @@ def test_apply_first_derivatives_host_bit_exact(fs):
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 def test_2000_steps(kind):
-    # the north star's parity length: 2000 steps from a smooth state, identical to the oracle
+    # the parity length the propagators are held to: 2000 steps from a smooth state, identical to
+    # the oracle
     import wavefd_amd.prop2d as w2
     if kind == "iso":
         prop, orc = _pair((40, 44), True, 41)
