@@ -108,26 +108,23 @@ __device__ __forceinline__ long block_elem(const QGeo &g, long blk, int e, bool 
     return in ? (i3 * g.n2 + i2) * g.n1 + i1 : 0;
 }
 
-// One wavefront per 8^3 block: lane t owns the block's row t (i2 = 8 k2 + t % 8, i3 = 8 k3 + t / 8),
-// i.e. its elements e = 8 t .. 8 t + 7 (e = (i1 & 7) + 8 (i2 & 7) + 64 (i3 & 7)); the block's
-// coordinates are formed once, the maximum is a wave reduction, and the base-11 digits pass
-// through LDS between the row layout and the 57 payload words (9 digits each, e = 9 w + j).
-__device__ __forceinline__ long row_base(const QGeo &g, long blk, int t, long &k1, bool &row_in) {
-    k1 = blk % g.m1;
-    const long rest = blk / g.m1, k2 = rest % g.m2, k3 = rest / g.m2;
-    const long i2 = k2 * BS + (t & 7), i3 = k3 * BS + (t >> 3);
-    row_in = i2 < g.n2 && i3 < g.n3;
-    return row_in ? (i3 * g.n2 + i2) * g.n1 + k1 * BS : 0;
-}
-
-__global__ void __launch_bounds__(64) k_encode(QGeo g, const float *__restrict__ x, uint32_t *__restrict__ code) {
-    const long blk = blockIdx.x;
-    const int t = threadIdx.x;
-    __shared__ uint8_t dig[BV];
-    long k1;
-    bool row_in;
-    const long base = row_base(g, blk, t, k1, row_in);
-    const long nrow = g.n1 - k1 * BS;  // elements of the row inside the grid
+// Four blocks along n1 per workgroup, the layout of k_dequantize: lane tid reads the 8 values of
+// block tid % 4 in row tid / 4 (four lanes, one 128-byte line); each block's maximum is a shuffle
+// over the 16 lanes of a wave that hold it, then over the 4 waves; one wave packs each block.
+__global__ void __launch_bounds__(256) k_encode(QGeo g, const float *__restrict__ x, uint32_t *__restrict__ code) {
+    const long m1q = (g.m1 + 3) / 4;
+    const long wg = blockIdx.x;
+    const long k1q = wg % m1q, rest = wg / m1q, k2 = rest % g.m2, k3 = rest / g.m2;
+    const int tid = threadIdx.x, w = tid >> 6, t = tid & 63;
+    const int q = tid & 3, r = tid >> 2;
+    __shared__ uint8_t dig[4][BV];
+    __shared__ float red[4][4];
+    __shared__ float sc[4];
+    const long k1 = k1q * 4 + q;
+    const long i2 = k2 * BS + (r & 7), i3 = k3 * BS + (r >> 3);
+    const bool row_in = k1 < g.m1 && i2 < g.n2 && i3 < g.n3;
+    const long base = row_in ? (i3 * g.n2 + i2) * g.n1 + k1 * BS : 0;
+    const long nrow = g.n1 - k1 * BS;
     float v[BS];
     float m = 0.0f;
 #pragma unroll
@@ -136,30 +133,37 @@ __global__ void __launch_bounds__(64) k_encode(QGeo g, const float *__restrict__
         m = fmaxf(m, fabsf(v[j]));
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    const uint16_t sb = f2bf_up(m);
-    const float s = bf2f(sb);
+    for (int off = 4; off < 64; off <<= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if (t < 4) red[w][t] = m;  // lane t < 4 holds block t's maximum over this wave
+    __syncthreads();
+    if (tid < 4) {
+        const float mm = fmaxf(fmaxf(red[0][tid], red[1][tid]), fmaxf(red[2][tid], red[3][tid]));
+        sc[tid] = bf2f(f2bf_up(mm));
+    }
+    __syncthreads();
+    const float s = sc[q];
 #pragma unroll
     for (int j = 0; j < BS; j++) {
-        int q = 0;
+        int qv = 0;
         if (s > 0.0f) {
-            q = (int)rintf(v[j] / s * (float)LV);
-            q = q < -LV ? -LV : (q > LV ? LV : q);
+            qv = (int)rintf(v[j] / s * (float)LV);
+            qv = qv < -LV ? -LV : (qv > LV ? LV : qv);
         }
-        dig[t * BS + j] = (uint8_t)(q + LV);
+        dig[q][r * BS + j] = (uint8_t)(qv + LV);
     }
-    uint16_t *scales = reinterpret_cast<uint16_t *>(code + 8);
-    if (t == 0) scales[blk] = sb;  // the largest of them goes to header word 7 (k_norm)
     __syncthreads();
-    uint32_t *pay = code + pay_off(g) + WPB * blk;
+    const long k1w = k1q * 4 + w;  // the block wave w packs
+    if (k1w >= g.m1) return;
+    const long blk = (k3 * g.m2 + k2) * g.m1 + k1w;
+    if (t == 0) reinterpret_cast<uint16_t *>(code + 8)[blk] = f2bf_up(sc[w]);  // the largest goes to word 7 (k_norm)
     if (t < WPB) {
-        uint32_t w = 0, p = 1;
+        uint32_t wd = 0, p = 1;
         for (int j = 0; j < 9; j++) {
             const int e = t * 9 + j;
-            w += (e < BV ? (uint32_t)dig[e] : (uint32_t)LV) * p;
+            wd += (e < BV ? (uint32_t)dig[w][e] : (uint32_t)LV) * p;
             p *= 11u;
         }
-        pay[t] = w;
+        code[pay_off(g) + WPB * blk + t] = wd;
     }
 }
 
@@ -209,30 +213,44 @@ __global__ void __launch_bounds__(1024) k_norm(QGeo g, uint32_t *__restrict__ co
     if (threadIdx.x == 0) code[7] = red[0] << 16;
 }
 
-__global__ void __launch_bounds__(64) k_dequantize(QGeo g, const uint32_t *__restrict__ code, float *__restrict__ out) {
-    const long blk = blockIdx.x;
-    const int t = threadIdx.x;
-    __shared__ uint8_t dig[9 * WPB];
+// Four blocks along n1 per workgroup (one wave unpacks each block's 57 words into LDS), then the
+// stores go row by row: lane tid writes the 8 values of block tid % 4 in row tid / 4, so four
+// consecutive lanes write 32 consecutive floats of the grid (one 128-byte line) instead of one
+// 32-byte piece per lane.
+__global__ void __launch_bounds__(256) k_dequantize(QGeo g, const uint32_t *__restrict__ code, float *__restrict__ out) {
+    const long m1q = (g.m1 + 3) / 4;
+    const long wg = blockIdx.x;
+    const long k1q = wg % m1q, rest = wg / m1q, k2 = rest % g.m2, k3 = rest / g.m2;
+    const int tid = threadIdx.x, w = tid >> 6, t = tid & 63;
+    __shared__ uint8_t dig[4][9 * WPB];
+    __shared__ float sc[4];
     const uint16_t *scales = reinterpret_cast<const uint16_t *>(code + 8);
-    const float s = bf2f(scales[blk]);
-    const uint32_t *pay = code + pay_off(g) + WPB * blk;
-    if (t < WPB) {
-        uint32_t w = pay[t];
-        for (int j = 0; j < 9; j++) {
-            dig[t * 9 + j] = (uint8_t)(w % 11u);
-            w /= 11u;
+    const long k1w = k1q * 4 + w;  // the block wave w unpacks
+    if (k1w < g.m1) {
+        const long blk = (k3 * g.m2 + k2) * g.m1 + k1w;
+        if (t == 0) sc[w] = bf2f(scales[blk]);
+        if (t < WPB) {
+            uint32_t word = code[pay_off(g) + WPB * blk + t];
+            for (int j = 0; j < 9; j++) {
+                dig[w][t * 9 + j] = (uint8_t)(word % 11u);
+                word /= 11u;
+            }
         }
     }
     __syncthreads();
-    long k1;
-    bool row_in;
-    const long base = row_base(g, blk, t, k1, row_in);
+    const int q = tid & 3, r = tid >> 2;  // block k1q * 4 + q, row r of it
+    const long k1 = k1q * 4 + q;
+    const long i2 = k2 * BS + (r & 7), i3 = k3 * BS + (r >> 3);
+    if (k1 >= g.m1 || i2 >= g.n2 || i3 >= g.n3) return;
+    const long base = (i3 * g.n2 + i2) * g.n1 + k1 * BS;
     const long nrow = g.n1 - k1 * BS;
-    if (!row_in) return;
+    const float s = sc[q];
 #pragma unroll
     for (int j = 0; j < BS; j++)
-        if (j < nrow) out[base + j] = dequant(s, dig[t * BS + j]);
+        if (j < nrow) out[base + j] = dequant(s, dig[q][r * BS + j]);
 }
+
+static inline unsigned dequantize_grid(const QGeo &g) { return (unsigned)(((g.m1 + 3) / 4) * g.m2 * g.m3); }
 
 // encoder's choice per block: flag the block when the learned decode o is closer to the exact x
 // than the plain dequantisation qh (sums of squares over the block's values)
@@ -704,7 +722,7 @@ long wavefd_lcodec_encode(const float *x, long n1, long n2, long n3, uint32_t *c
     k_header<<<1, 64, 0, s>>>(code, (uint32_t)n1, (uint32_t)n2, (uint32_t)n3);
     LC_TRY(hipGetLastError());
     LC_TRY(hipMemsetAsync(code + 8, 0, sizeof(uint32_t) * (pay_off(g) - 8), s));  // scales and flags
-    k_encode<<<(unsigned)g.nblocks(), 64, 0, s>>>(g, x, code);
+    k_encode<<<(unsigned)(((g.m1 + 3) / 4) * g.m2 * g.m3), 256, 0, s>>>(g, x, code);
     LC_TRY(hipGetLastError());
     k_norm<<<1, 1024, 0, s>>>(g, code);
     LC_TRY(hipGetLastError());
@@ -744,7 +762,7 @@ long wavefd_lcodec_dequantize(const uint32_t *code, float *out, void *stream) {
     long n[3];
     if (read_header(code, n, s)) return -1;
     const QGeo g = qgeo(n[0], n[1], n[2]);
-    k_dequantize<<<(unsigned)g.nblocks(), 64, 0, s>>>(g, code, out);
+    k_dequantize<<<dequantize_grid(g), 256, 0, s>>>(g, code, out);
     LC_TRY(hipGetLastError());
     return 0;
 }
@@ -756,7 +774,7 @@ long wavefd_lcodec_dequantize_n(const uint32_t *code, long n1, long n2, long n3,
         return -1;
     }
     const QGeo g = qgeo(n1, n2, n3);
-    k_dequantize<<<(unsigned)g.nblocks(), 64, 0, (hipStream_t)stream>>>(g, code, out);
+    k_dequantize<<<dequantize_grid(g), 256, 0, (hipStream_t)stream>>>(g, code, out);
     LC_TRY(hipGetLastError());
     return 0;
 }
