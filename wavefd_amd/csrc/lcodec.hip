@@ -668,15 +668,45 @@ void host_mask(std::vector<float> &mk) {
 // out = ca a + cb b + cc c, b and c optional, evaluated in that order: the combinations of a
 // closed-loop temporal predictor (prediction 2 a - b, residual x - prediction, reconstruction
 // prediction + decoded residual), the same kernel on the encoding and the decoding side
+// (16-byte accesses over the first n & ~3 elements -- the arrays are whole device allocations,
+// 256-byte aligned -- then the last n % 4 one by one; the same arithmetic per element)
+__device__ __forceinline__ float lin3(float ca, float a, float cb, const float *b, long i, float cc, const float *c) {
+    float v = ca * a;
+    if (b) v = v + cb * b[i];
+    if (c) v = v + cc * c[i];
+    return v;
+}
+
 __global__ void k_lincomb(long n, float ca, const float *__restrict__ a, float cb, const float *__restrict__ b,
                           float cc, const float *__restrict__ c, float *__restrict__ out) {
     const long stride = (long)gridDim.x * blockDim.x;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        float v = ca * a[i];
-        if (b) v = v + cb * b[i];
-        if (c) v = v + cc * c[i];
-        out[i] = v;
+    const long n4 = n >> 2;
+    for (long i4 = (long)blockIdx.x * blockDim.x + threadIdx.x; i4 < n4; i4 += stride) {
+        const float4 av = reinterpret_cast<const float4 *>(a)[i4];
+        float4 bv = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cv = bv;
+        if (b) bv = reinterpret_cast<const float4 *>(b)[i4];
+        if (c) cv = reinterpret_cast<const float4 *>(c)[i4];
+        float4 o;
+        o.x = ca * av.x;
+        o.y = ca * av.y;
+        o.z = ca * av.z;
+        o.w = ca * av.w;
+        if (b) {
+            o.x = o.x + cb * bv.x;
+            o.y = o.y + cb * bv.y;
+            o.z = o.z + cb * bv.z;
+            o.w = o.w + cb * bv.w;
+        }
+        if (c) {
+            o.x = o.x + cc * cv.x;
+            o.y = o.y + cc * cv.y;
+            o.z = o.z + cc * cv.z;
+            o.w = o.w + cc * cv.w;
+        }
+        reinterpret_cast<float4 *>(out)[i4] = o;
     }
+    for (long i = n4 * 4 + (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = lin3(ca, a[i], cb, b, i, cc, c);
 }
 
 // the code's header (magic, grid, block size, levels, digits per word; word 7, the largest
@@ -701,7 +731,7 @@ long wavefd_lcodec_combine(long n, float ca, const float *a, float cb, const flo
         return -1;
     }
     if (n == 0) return 0;
-    const long blocks = std::min<long>((n + 255) / 256, 1L << 20);
+    const long blocks = std::max<long>(1, std::min<long>((n / 4 + 255) / 256, 1L << 20));
     k_lincomb<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(n, ca, a, cb, b, cc, c, out);
     LC_TRY(hipGetLastError());
     return 0;
