@@ -95,3 +95,53 @@ def test_many_steps_device_mode(oracle, kind):
     w.sync_to_host(prop)
     names = ["pCur", "pOld"] + (["mCur", "mOld"] if kind != "iso" else [])
     _compare(prop, orc, kind, shape, names, exact=False)
+
+
+def _prop_only(kind, shape, seed):
+    import wavefd_amd as w
+    nx, ny, nz = shape
+    cls = {"iso": w.Prop3DAcoIsoDenQ_DEO2_FDTD, "vti": w.Prop3DAcoVTIDenQ_DEO2_FDTD,
+           "tti": w.Prop3DAcoTTIDenQ_DEO2_FDTD}[kind]
+    prop = cls(nz=nz, ny=ny, nx=nx, dz=8.0, dy=12.5, dx=10.0, dt=1e-3, nsponge=6, freesurface=True, freqQ=10.0,
+               qMin=0.5, qInterior=80.0)
+    for name, arr in list(make_model(kind, shape, seed).items()) + list(make_fields(kind, shape, seed).items()):
+        host_view(prop, name)[:] = arr.ravel()
+    return prop
+
+
+@pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
+def test_q_table_and_q_array_give_the_same_step(kind):
+    # while dtOmegaInvQ is the SetupDtOmegaInvQ profile the fused step reads Q from a table by
+    # sponge distance; once Q's device address is handed out it reads the array: same bits
+    import wavefd_amd as w
+    shape = (40, 33, 37)
+    a, b = _prop_only(kind, shape, 11), _prop_only(kind, shape, 11)
+    w.coherence(a, "device")
+    w.coherence(b, "device")
+    w.sync_to_device(a)
+    w.sync_to_device(b)
+    assert w.device_ptr(b, "dtOmegaInvQ")  # b: the array path from here on
+    for _ in range(4):
+        w.propagateforward(a)
+        w.propagateforward(b)
+    w.sync_to_host(a)
+    w.sync_to_host(b)
+    names = ["pCur", "pOld", "pSpace"] + (["mCur", "mOld", "mSpace"] if kind != "iso" else [])
+    for name in names:
+        assert np.array_equal(to_oracle(a, name, shape), to_oracle(b, name, shape)), name
+
+
+@pytest.mark.parametrize("kind", ["iso", "tti"])
+def test_written_q_is_used(oracle, kind):
+    # Q written through its host view (a caller's own attenuation model) is what the step uses
+    import wavefd_amd as w
+    shape = (37, 41, 45)
+    prop, orc = _pair(oracle, kind, shape, True, 13)
+    q = (0.002 + 0.01 * np.random.default_rng(5).random(shape)).astype(np.float32)
+    host_view(prop, "q")[:] = q.ravel()
+    orc.q[...] = q
+    for _ in range(3):
+        w.propagateforward(prop)
+        orc.time_step()
+    names = ["pCur", "pOld", "pSpace"] + (["mCur", "mOld", "mSpace"] if kind != "iso" else [])
+    _compare(prop, orc, kind, shape, names)

@@ -207,7 +207,9 @@ class _CheckpointStore:
     def _second(self):
         r = self.run
         p2 = r.new_prop()
-        for name in MODEL_FIELDS[r.physics] + ("dtOmegaInvQ",):
+        # the model; Q needs no copy: the same construction parameters give the same profile
+        # (and handing out Q's address would end the fused step's table lookup of it)
+        for name in MODEL_FIELDS[r.physics]:
             ginsu.copy(prop3d.device_ptr(p2, name), prop3d.device_ptr(r.prop, name), 4 * self.ncell,
                        p2._lib.GetStream(p2.p))
         prop3d.set_sources(p2, [r.src_pts], r.shot.wavelet, fields=1 if self.nf == 1 else 3)

@@ -50,6 +50,10 @@ struct FusedArgs {
     const float *pCur, *mCur;
     float *pOld, *mOld, *pSpace, *mSpace;
     const float *v, *b, *q, *eps, *eta, *f, *sT, *cT, *sP, *cP;
+    // Q by sponge distance (nsponge + 1 values) while q holds the SetupDtOmegaInvQ profile, else
+    // null: the update then takes qtab[min(distance, nsponge)] instead of loading q
+    const float *qtab;
+    int nsponge;
     // epilogue operands (see Epilogue)
     float dt;
     float *e_dp, *e_dm;             // SAVE_DP: DP, DM out; IMAGE: g (e_dp) in/out
@@ -205,9 +209,24 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<fu
     }
 
     const long nynz = g.nynz;
+    // this column's sponge distance in y and z (k_setup_q's rule); the x part is added per plane.
+    // Iso and VTI only: TTI's step runs at the register limit, and the extra live value and
+    // branch cost it more (12.7 -> 13.0 ms at 800^3) than the table saves
+    constexpr bool QT = PHYS != TTI;
+    const int kyz = QT ? min(min(y, ny - 1 - y), g.fs ? nz - 1 - z : min(z, nz - 1 - z)) : 0;
     // loads are issued raw; the zeroing outside the grid and the mirror's sign are applied
     // when the value is consumed, so that no wait follows a load
     auto ld = [&](const float *a, int xx, uint32_t off) -> float { return gld(plane(a, xx, nx, nynz), off); };
+    // output operand k of plane xx: Q from the table while it is the profile (the index clamped
+    // into the table for columns and planes whose value is never used)
+    auto opnd = [&](int k, int xx, uint32_t off) -> float {
+        if (QT && k == 1 && A.qtab) {
+            int ks = min(min(xx, nx - 1 - xx), kyz);
+            ks = ks < 0 ? 0 : (ks > A.nsponge ? A.nsponge : ks);
+            return A.qtab[ks];
+        }
+        return ld(oin[k], xx, off);
+    };
     uint32_t own_s, boff_s, moff_s, ooff_s;  // this step's copies of the offsets
     auto launder_all = [&]() {
         own_s = launder(own);
@@ -248,7 +267,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<fu
 #pragma unroll
     for (int k = 0; k < NM; k++) mnx[k] = ld(mat[k], xfirst + 3, moff_s);
 #pragma unroll
-    for (int k = 0; k < NO; k++) onx[k] = ld(oin[k], xfirst, ooff_s);
+    for (int k = 0; k < NO; k++) onx[k] = opnd(k, xfirst, ooff_s);
 
     auto iter = [&](auto U, int x) {
         constexpr int u = decltype(U)::value;  // x - xfirst = u (mod 4)
@@ -278,7 +297,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<fu
 #pragma unroll
         for (int k = 0; k < NM; k++) mnx[k] = ld(mat[k], x + 4, moff_s);
 #pragma unroll
-        for (int k = 0; k < NO; k++) onx[k] = ld(oin[k], x + 1, ooff_s);
+        for (int k = 0; k < NO; k++) onx[k] = opnd(k, x + 1, ooff_s);
         __syncthreads();
         // ---- 3. gradient + material sandwich at (x+3, y, z) ----------------------------
         const int xp = x + 3;
@@ -617,6 +636,8 @@ static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float 
     A.v = f[F_V].d;
     A.b = f[F_B].d;
     A.q = f[F_Q].d;
+    A.qtab = (PHYS != TTI && p->q_profile) ? p->qtab : nullptr;
+    A.nsponge = (int)p->nsponge;
     A.eps = f[F_EPS].d;
     A.eta = f[F_ETA].d;
     A.f = f[F_F].d;

@@ -189,6 +189,11 @@ struct Prop {
     hipEvent_t ev = nullptr;         // orders this handle's stream against a batch's (lazy)
     bool born_field = false;         // Iso: mCur/mOld hold the coupled Born step's second field
     long *it_dev = nullptr;          // device time index of graph-replayed loops (lazy)
+    // Q by sponge distance (nsponge + 1 values, the last the interior's) while dtOmegaInvQ is
+    // the SetupDtOmegaInvQ profile: the fused step then reads the table instead of the array.
+    // Handing out Q's host mirror or device address (a caller may write it) ends that.
+    float *qtab = nullptr;
+    bool q_profile = false;
     int dim = 3;                     // 2: a 2D propagator (ny = 1, two-pass 2D kernels)
 
     Mask model_mask() const {
@@ -232,6 +237,7 @@ struct Prop {
     ~Prop() {
         if (stream) hipStreamSynchronize(stream);
         if (fused_sync) (void)hipFree(fused_sync);
+        if (qtab) (void)hipFree(qtab);
         if (it_dev) (void)hipFree(it_dev);
         if (ev) (void)hipEventDestroy(ev);
         for (int i = 0; i < F_COUNT; i++) {
@@ -265,6 +271,7 @@ struct Prop {
     // ---- host mirrors ------------------------------------------------------------------
     float *host(int i) {
         if (!f[i].used) return nullptr;
+        if (i == F_Q) q_profile = false;  // the caller may write Q through the mirror
         if (!f[i].h) {
             if (hipHostMalloc(&f[i].h, sizeof(float) * n, hipHostMallocDefault) != hipSuccess) {
                 (void)hipGetLastError();
@@ -394,6 +401,7 @@ static inline bool setup_q(Prop *p, float freqQ, float qMin, float qInterior) {
         std::printf("Warning -- qMin and qMax unphysical, dtOmegaInvQ set to zero!\n");
         // the reference clears sizeof(float*) bytes, i.e. the first two values (src/absorb.h:107)
         const long nzero = std::min<long>(2, p->n);
+        p->q_profile = false;
         WFD_TRY(hipMemsetAsync(p->f[F_Q].d, 0, sizeof(float) * nzero, p->stream));
         return p->end(bit(F_Q), true);
     }
@@ -413,6 +421,15 @@ static inline bool setup_q(Prop *p, float freqQ, float qMin, float qInterior) {
     WFD_LAUNCHED();
     WFD_TRY(hipStreamSynchronize(p->stream));
     hipFree(dq);
+    // the same values by sponge distance (k_setup_q's expression, IEEE double on both sides)
+    std::vector<float> qt(ns + 1);
+    const double PI_REF = 3.1415926535897;
+    const double num = (double)(p->dt * 2) * PI_REF * (double)freqQ;
+    for (long ks = 0; ks < ns; ks++) qt[ks] = (float)(num / (double)qprof[ks]);
+    qt[ns] = (float)(num / (double)qInterior);
+    if (!p->qtab) WFD_TRY(hipMalloc(&p->qtab, sizeof(float) * (ns + 1)));
+    WFD_TRY(hipMemcpy(p->qtab, qt.data(), sizeof(float) * (ns + 1), hipMemcpyHostToDevice));
+    p->q_profile = true;
     return p->end(bit(F_Q), true);
 }
 
