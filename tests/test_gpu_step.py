@@ -138,7 +138,11 @@ def test_written_q_is_used(oracle, kind):
     shape = (37, 41, 45)
     prop, orc = _pair(oracle, kind, shape, True, 13)
     q = (0.002 + 0.01 * np.random.default_rng(5).random(shape)).astype(np.float32)
-    host_view(prop, "q")[:] = q.ravel()
+    view = host_view(prop, "q")
+    # a later SetupDtOmegaInvQ does not bring the table back: the view handed out may still be
+    # written, and is
+    prop._lib.SetupDtOmegaInvQ(prop.p, 10.0, 0.5, 80.0)
+    view[:] = q.ravel()
     orc.q[...] = q
     for _ in range(3):
         w.propagateforward(prop)

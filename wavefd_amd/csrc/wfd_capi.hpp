@@ -68,7 +68,7 @@
     float *WFD_CAT(Prefix, GetDevicePtr)(void *p, const char *name) {                                     \
         wfd::Prop *pp = reinterpret_cast<wfd::Prop *>(p);                                                 \
         const int i = wfd::field_by_name(name);                                                           \
-        if (i == wfd::F_Q) pp->q_profile = false; /* the caller may write Q there */                     \
+        if (i == wfd::F_Q) pp->q_profile = false, pp->q_exposed = true; /* the caller may write Q there */ \
         return (i >= 0 && pp->f[i].used) ? pp->f[i].d : nullptr;                                          \
     }                                                                                                     \
     void *WFD_CAT(Prefix, GetStream)(void *p) { return (void *)reinterpret_cast<wfd::Prop *>(p)->stream; } \

@@ -191,9 +191,11 @@ struct Prop {
     long *it_dev = nullptr;          // device time index of graph-replayed loops (lazy)
     // Q by sponge distance (nsponge + 1 values, the last the interior's) while dtOmegaInvQ is
     // the SetupDtOmegaInvQ profile: the fused step then reads the table instead of the array.
-    // Handing out Q's host mirror or device address (a caller may write it) ends that.
+    // Handing out Q's host mirror or device address (a caller may write it, now or later) ends
+    // that for good.
     float *qtab = nullptr;
     bool q_profile = false;
+    bool q_exposed = false;
     int dim = 3;                     // 2: a 2D propagator (ny = 1, two-pass 2D kernels)
 
     Mask model_mask() const {
@@ -271,7 +273,7 @@ struct Prop {
     // ---- host mirrors ------------------------------------------------------------------
     float *host(int i) {
         if (!f[i].used) return nullptr;
-        if (i == F_Q) q_profile = false;  // the caller may write Q through the mirror
+        if (i == F_Q) q_profile = false, q_exposed = true;  // the caller may write Q through the mirror
         if (!f[i].h) {
             if (hipHostMalloc(&f[i].h, sizeof(float) * n, hipHostMallocDefault) != hipSuccess) {
                 (void)hipGetLastError();
@@ -429,7 +431,7 @@ static inline bool setup_q(Prop *p, float freqQ, float qMin, float qInterior) {
     qt[ns] = (float)(num / (double)qInterior);
     if (!p->qtab) WFD_TRY(hipMalloc(&p->qtab, sizeof(float) * (ns + 1)));
     WFD_TRY(hipMemcpy(p->qtab, qt.data(), sizeof(float) * (ns + 1), hipMemcpyHostToDevice));
-    p->q_profile = true;
+    p->q_profile = !p->q_exposed;
     return p->end(bit(F_Q), true);
 }
 
