@@ -25,7 +25,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-BYTES_PER_CELL = {"iso": 28, "vti": 56, "tti": 72}
+# Iso and VTI keep SetupDtOmegaInvQ's profile here and read Q from its table by sponge distance
+# (no Q stream: Iso 6 float streams, VTI 13); TTI's workload writes its own Q (18 streams)
+BYTES_PER_CELL = {"iso": 24, "vti": 52, "tti": 72}
 HBM_PEAK_GBS = 8000.0
 REFERENCE_44T_MCELLS = {"iso": 1961.90, "vti": 893.94, "tti": 800.58}  # docs/src/index.md:42-44
 
@@ -204,9 +206,10 @@ def run_gpu(physics, n, steps, warmup, seed=3, tune=False, with_taps=False):
     }
 
 
-# 2D steps: compulsory HBM bytes per cell (Iso: pCur, pOld, v, b, Q in, p+, pSpace out; VTI adds
-# mCur, mOld, eps, eta, f in and m+, mSpace out; TTI adds sin/cos theta)
-BYTES_2D = {"iso": 28, "vti": 56, "tti": 64}
+# 2D steps: compulsory HBM bytes per cell (Iso: pCur, pOld, v, b in, p+, pSpace out; VTI adds
+# mCur, mOld, eps, eta, f in and m+, mSpace out; TTI adds sin/cos theta); Q is the profile and
+# comes from its table by sponge distance
+BYTES_2D = {"iso": 24, "vti": 52, "tti": 60}
 
 
 def run_2d(physics, n, steps, warmup):

@@ -371,3 +371,45 @@ def test_apply_first_derivatives_host_caller_coefficients(fs):
             wz[kx, kz] = idz * sz
     assert np.array_equal(ox, wx)
     assert np.array_equal(oz, wz)
+
+
+@pytest.mark.parametrize("fs", [False, True])
+@pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
+def test_q_table_and_q_array_give_the_same_step(kind, fs):
+    # while DtOmegaInvQ is the SetupDtOmegaInvQ profile the fused step reads Q from a table by
+    # sponge distance; once Q's host view or device address is handed out it reads the array:
+    # the same bits, the oracle's
+    import wavefd_amd.prop2d as w2
+    shape = (100, 300)
+    mk = (lambda: _pair(shape, fs, 57)) if kind == "iso" else (lambda: _apair(kind, shape, fs, 57))
+    (a, orc), (b, _), (c, _) = mk(), mk(), mk()
+    _get(b, "DtOmegaInvQ")  # b: the array path from here on
+    assert w2.device_ptr(c, "dtOmegaInvQ")  # c: likewise
+    for _ in range(7):
+        for prop in (a, b, c):
+            w2.propagateforward(prop)
+        orc.time_step()
+    names = ("pCur", "pOld", "pSpace") + (("mCur", "mOld", "mSpace") if kind != "iso" else ())
+    for name in names:
+        got = [_get(prop, _GET[name]) for prop in (a, b, c)]
+        assert np.array_equal(got[0], getattr(orc, name)), name
+        assert np.array_equal(got[1], got[0]) and np.array_equal(got[2], got[0]), name
+
+
+def test_written_q_is_used():
+    # Q written through its device address after the profile (a caller's own attenuation model)
+    # is what the fused step uses
+    import wavefd_amd.prop2d as w2
+    from wavefd_amd import ginsu
+    shape = (100, 300)
+    prop, orc = _pair(shape, True, 59)
+    q = (0.002 + 0.01 * np.random.default_rng(5).random(shape)).astype(F32)
+    ptr = w2.device_ptr(prop, "dtOmegaInvQ")
+    ginsu.copy(ptr, q.ctypes.data, q.nbytes)
+    ginsu.sync()
+    orc.q[...] = q
+    for _ in range(5):
+        w2.propagateforward(prop)
+        orc.time_step()
+    for gname, oname in (("PCur", "pCur"), ("POld", "pOld"), ("PSpace", "pSpace")):
+        assert np.array_equal(_get(prop, gname), getattr(orc, oname)), gname

@@ -83,7 +83,9 @@ void Prop2DAcoTTIDenQ_DEO2_FDTD_SyncToHost(void *p) {
 }
 float *Prop2DAcoTTIDenQ_DEO2_FDTD_GetDevicePtr(void *p, const char *name) {
     const int i = wfd::field_by_name(name);
-    return (i >= 0 && P2(p)->f[i].used) ? P2(p)->f[i].d : nullptr;
+    if (i < 0 || !P2(p)->f[i].used) return nullptr;
+    if (i == wfd::F_Q) P2(p)->q_profile = false, P2(p)->q_exposed = true;  // the caller may write Q there
+    return P2(p)->f[i].d;
 }
 void Prop2DAcoTTIDenQ_DEO2_FDTD_SetFusedStep(void *p, long fused) { P2(p)->fused = fused ? 1 : 0; }
 // additive: the Born loops' per-step pairs as one step (same results as the two calls)

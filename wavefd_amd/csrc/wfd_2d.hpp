@@ -506,13 +506,15 @@ static inline bool step_2d(Prop *p, bool linear) {
 }
 
 // the 2D Q profile (src/absorb.h:23-75): the sponge ramp in double, exp in float, the stored
-// value a double expression; computed on the host once and uploaded
+// value a double expression; computed on the host once and uploaded, the array and the table by
+// sponge distance (Prop::qtab, nsponge + 1 values) from the same values
 static inline bool setup_q_2d(Prop *p, float freqQ, float qMin, float qInterior) {
     if (!p->begin(bit(F_Q))) return false;
     if ((qMin < FLT_EPSILON) && (qInterior < FLT_EPSILON)) {
         std::printf("Warning -- qMin and qMax unphysical, dtOmegaInvQ set to zero!\n");
         // the reference clears sizeof(float*) bytes, i.e. the first two values (src/absorb.h:44)
         const long nzero = std::min<long>(2, p->n);
+        p->q_profile = false;
         WFD_TRY(hipMemsetAsync(p->f[F_Q].d, 0, sizeof(float) * nzero, p->stream));
         return p->end(bit(F_Q), true);
     }
@@ -525,18 +527,21 @@ static inline bool setup_q_2d(Prop *p, float freqQ, float qMin, float qInterior)
         qprof[ks] = expf((float)lq);
     }
     const double pi = 3.1415926535897;
+    std::vector<float> qt(ns + 1);
+    for (long ks = 0; ks < ns; ks++) qt[ks] = (float)(p->dt * 2.0 * pi * freqQ / qprof[ks]);
+    qt[ns] = (float)(p->dt * 2.0 * pi * freqQ / qInterior);
     std::vector<float> q((size_t)p->n);
     for (long kx = 0; kx < nx; kx++)
         for (long kz = 0; kz < nz; kz++) {
             const long ksx = std::min(kx, nx - 1 - kx);
             const long ksz = p->fs ? (nz - 1 - kz) : std::min(kz, nz - 1 - kz);
-            const long ks = std::min(ksx, ksz);
-            float val = (float)(p->dt * 2.0 * pi * freqQ / qInterior);
-            if (ks < ns) val = (float)(p->dt * 2.0 * pi * freqQ / qprof[ks]);
-            q[kx * nz + kz] = val;
+            q[kx * nz + kz] = qt[std::min(std::min(ksx, ksz), ns)];
         }
+    if (!p->qtab) WFD_TRY(hipMalloc(&p->qtab, sizeof(float) * (ns + 1)));
+    WFD_TRY(hipMemcpyAsync(p->qtab, qt.data(), sizeof(float) * (ns + 1), hipMemcpyHostToDevice, p->stream));
     WFD_TRY(hipMemcpyAsync(p->f[F_Q].d, q.data(), sizeof(float) * p->n, hipMemcpyHostToDevice, p->stream));
     WFD_TRY(hipStreamSynchronize(p->stream));
+    p->q_profile = !p->q_exposed;
     return p->end(bit(F_Q), true);
 }
 

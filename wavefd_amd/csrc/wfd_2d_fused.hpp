@@ -50,6 +50,10 @@ struct Fused2Args {
     float dt;
     float *e_dp, *e_dm;
     const float *e_dv, *e_ip, *e_im;
+    // Q by sponge distance (nsponge + 1 values) while Q is the setup profile, else null: the
+    // update then takes qtab[min(distance, nsponge)] instead of loading q
+    const float *qtab;
+    int nsponge;
 };
 
 template <int PHYS>
@@ -150,6 +154,17 @@ __device__ __forceinline__ void fused2_body(const Fused2Args<PHYS> &A, Fused2Lds
         const int xcl = xx < 0 ? 0 : (xx >= nx ? nx - 1 : xx);
         return a[(long)xcl * nzl + off];
     };
+    // output operand k of plane xx: Q from the table while it is the profile (the index clamped
+    // into the table for cells whose value is never used)
+    const int kzs = FST ? nz - 1 - z : min(z, nz - 1 - z);
+    auto opnd = [&](int k, int xx, int off) -> float {
+        if (k == 1 && A.qtab) {
+            int ks = min(min(xx, nx - 1 - xx), kzs);
+            ks = ks < 0 ? 0 : (ks > A.nsponge ? A.nsponge : ks);
+            return A.qtab[ks];
+        }
+        return ld(oin[k], xx, off);
+    };
     auto fixp = [&](float v, int xx) -> float {
         v = (own_ok && xx >= 0 && xx < nx) ? v : 0.0f;
         if constexpr (FST) v = flip(v, own_sign);
@@ -181,7 +196,7 @@ __device__ __forceinline__ void fused2_body(const Fused2Args<PHYS> &A, Fused2Lds
 #pragma unroll
     for (int k = 0; k < NM; k++) mnx[k] = ld(mat[k], xfirst + 3, moff);
 #pragma unroll
-    for (int k = 0; k < NO; k++) onx[k] = ld(oin[k], xfirst, ooff);
+    for (int k = 0; k < NO; k++) onx[k] = opnd(k, xfirst, ooff);
 
     auto iter = [&](auto U, int x) {
         constexpr int u = decltype(U)::value;  // x - xfirst = u (mod 4)
@@ -210,7 +225,7 @@ __device__ __forceinline__ void fused2_body(const Fused2Args<PHYS> &A, Fused2Lds
 #pragma unroll
         for (int k = 0; k < NM; k++) mnx[k] = ld(mat[k], x + 4, moff);
 #pragma unroll
-        for (int k = 0; k < NO; k++) onx[k] = ld(oin[k], x + 1, ooff);
+        for (int k = 0; k < NO; k++) onx[k] = opnd(k, x + 1, ooff);
         __syncthreads();
         // ---- 3. sandwich at (x+3, z) ---------------------------------------------------
         const int xp = x + 3;
@@ -426,6 +441,8 @@ static inline bool launch_fused2(Prop *p, bool linear, float *e0 = nullptr, floa
     A.v = f[F_V].d;
     A.b = f[F_B].d;
     A.q = f[F_Q].d;
+    A.qtab = p->q_profile ? p->qtab : nullptr;
+    A.nsponge = (int)p->nsponge;
     A.eps = PHYS == ISO ? nullptr : f[F_EPS].d;
     A.eta = PHYS == ISO ? nullptr : f[F_ETA].d;
     A.f = PHYS == ISO ? nullptr : f[F_F].d;
