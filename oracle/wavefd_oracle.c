@@ -730,8 +730,10 @@ int wo2_setup_q(long fs, long nx, long nz, long nsponge, float dt, float freqQ, 
         return 0;
     }
     float *qprof = (float *)malloc(sizeof(float) * (nsponge > 0 ? nsponge : 1));
-    const double lqmin = log((double)qMin);
-    const double lqmax = log((double)qInterior);
+    /* the reference is C++: log(qMin) on a float picks the float overload (logf), and the
+     * result is widened to double for the ramp (src/absorb.h:47-48) */
+    const double lqmin = (double)logf(qMin);
+    const double lqmax = (double)logf(qInterior);
     for (long ks = 0; ks < nsponge; ks++) {
         const double dk = (double)(ks) / (double)(nsponge - 1);
         const double lq = lqmin + dk * (lqmax - lqmin);

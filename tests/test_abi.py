@@ -127,3 +127,18 @@ def test_product_does_not_reference_oracle():
                 if f.endswith((".py", ".hip", ".hpp", ".h", ".cpp")):
                     text = open(os.path.join(dirpath, f), errors="replace").read()
                     assert "oracle" not in text.lower(), os.path.join(dirpath, f)
+
+
+@pytest.mark.parametrize("kind", ["iso", "vti", "tti", "iso2d", "vti2d", "tti2d"])
+def test_setup_q_rejects_small_freq_without_exiting(kind):
+    """freqQ < FLT_EPSILON: the reference prints its message and calls exit(EXIT_FAILURE)
+    (src/absorb.h:34-39, :98-103).  The drop-in prints the same message, records it for
+    LastError and returns, leaving the handle alone (it is never dereferenced here: a null
+    handle proves that, and no GPU is needed)."""
+    lib = _lib.load(kind)
+    lib.ClearError()
+    lib.SetupDtOmegaInvQ(None, 0.0, 0.1, 100.0)
+    msg = _lib.last_error(kind)
+    assert "freqQ [0.000000] is too small" in msg
+    lib.ClearError()
+    assert _lib.last_error(kind) == ""

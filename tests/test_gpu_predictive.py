@@ -67,12 +67,31 @@ def _frames(shape, nt, seed):
     return out
 
 
-@pytest.mark.parametrize("order,learned", [(1, False), (2, False), (3, False), (3, True)])
-def test_round_trip_bounds_and_reverse_decode(order, learned):
+def test_combine_unaligned_slots_match_numpy_float32():
+    # segment slots of a frame with an odd cell count start at 4-byte offsets; the combine then
+    # takes its element-by-element form, with the same bits
+    from wavefd_amd import ginsu, lcodec
+    rng = np.random.default_rng(1)
+    n = 4 * 2_001 + 1
+    a, b, c = (rng.standard_normal(n).astype(F32) for _ in range(3))
+    da, db, dc = _dev(np.r_[F32(0), a]), _dev(np.r_[F32(0), F32(0), b]), _dev(np.r_[F32(0), F32(0), F32(0), c])
+    out = ginsu.DeviceBuffer(4 * (n + 1))
+    for ca, cb, cc in ((2.0, -1.0, 0.0), (0.5, -3.0, 1.25)):
+        lcodec.combine(n, ca, da.ptr + 4, out.ptr + 4, cb, db.ptr + 8, cc, dc.ptr + 12)
+        ref = (F32(ca) * a + F32(cb) * b + F32(cc) * c).astype(F32)
+        assert np.array_equal(_get(out.ptr + 4, n), ref)
+    for x in (da, db, dc, out):
+        x.free()
+
+
+@pytest.mark.parametrize("order,learned,shape", [(1, False, (40, 36, 32)), (2, False, (40, 36, 32)),
+                                                 (3, False, (40, 36, 32)), (3, True, (40, 36, 32)),
+                                                 (3, False, (39, 35, 31))])
+def test_round_trip_bounds_and_reverse_decode(order, learned, shape):
     # learned: residuals coded by a (seeded, untrained) learned decoder, whose correction stays
-    # inside the quantisation cell, so the same bounds hold
+    # inside the quantisation cell, so the same bounds hold.  (39, 35, 31): an odd cell count,
+    # whose segment slots are only 4-byte aligned
     from wavefd_amd import _lib, born, ginsu, lcodec
-    shape = (40, 36, 32)  # (nz, ny, nx)
     n = int(np.prod(shape))
     nt, k = 37, 8
     frames = _frames(shape, nt, 1)

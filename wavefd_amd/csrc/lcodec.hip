@@ -668,13 +668,22 @@ void host_mask(std::vector<float> &mk) {
 // out = ca a + cb b + cc c, b and c optional, evaluated in that order: the combinations of a
 // closed-loop temporal predictor (prediction 2 a - b, residual x - prediction, reconstruction
 // prediction + decoded residual), the same kernel on the encoding and the decoding side
-// (16-byte accesses over the first n & ~3 elements -- the arrays are whole device allocations,
-// 256-byte aligned -- then the last n % 4 one by one; the same arithmetic per element)
+// (16-byte accesses over the first n & ~3 elements when every array is 16-byte aligned, then
+// the last n % 4 one by one; otherwise element by element -- the same arithmetic per element,
+// so the two forms give the same bits)
 __device__ __forceinline__ float lin3(float ca, float a, float cb, const float *b, long i, float cc, const float *c) {
     float v = ca * a;
     if (b) v = v + cb * b[i];
     if (c) v = v + cc * c[i];
     return v;
+}
+
+__global__ void k_lincomb_scalar(long n, float ca, const float *__restrict__ a, float cb,
+                                 const float *__restrict__ b, float cc, const float *__restrict__ c,
+                                 float *__restrict__ out) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = lin3(ca, a[i], cb, b, i, cc, c);
 }
 
 __global__ void k_lincomb(long n, float ca, const float *__restrict__ a, float cb, const float *__restrict__ b,
@@ -731,8 +740,17 @@ long wavefd_lcodec_combine(long n, float ca, const float *a, float cb, const flo
         return -1;
     }
     if (n == 0) return 0;
-    const long blocks = std::max<long>(1, std::min<long>((n / 4 + 255) / 256, 1L << 20));
-    k_lincomb<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(n, ca, a, cb, b, cc, c, out);
+    // segment slots of an odd-sized frame sit at 4-byte offsets: 16-byte accesses only when
+    // every array allows them
+    auto al16 = [](const void *q) { return ((uintptr_t)q & 15u) == 0; };
+    const bool vec = al16(a) && al16(out) && (!b || al16(b)) && (!c || al16(c));
+    if (vec) {
+        const long blocks = std::max<long>(1, std::min<long>((n / 4 + 255) / 256, 1L << 20));
+        k_lincomb<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(n, ca, a, cb, b, cc, c, out);
+    } else {
+        const long blocks = std::max<long>(1, std::min<long>((n + 255) / 256, 1L << 20));
+        k_lincomb_scalar<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(n, ca, a, cb, b, cc, c, out);
+    }
     LC_TRY(hipGetLastError());
     return 0;
 }

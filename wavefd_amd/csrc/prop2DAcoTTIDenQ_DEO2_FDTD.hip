@@ -17,12 +17,8 @@ void *Prop2DAcoTTIDenQ_DEO2_FDTD_alloc(long fs, long nthread, long nx, long nz, 
 void Prop2DAcoTTIDenQ_DEO2_FDTD_free(void *p) { delete P2(p); }
 
 void Prop2DAcoTTIDenQ_DEO2_FDTD_SetupDtOmegaInvQ(void *p, float freqQ, float qMin, float qInterior) {
-    if (freqQ < FLT_EPSILON) {  // the reference's failure behaviour (src/absorb.h:34-39)
-        char msg[1000];
-        std::snprintf(msg, sizeof(msg), "Error -- freqQ [%f] is too small!\n", freqQ);
-        std::perror(msg);
-        std::exit(EXIT_FAILURE);
-    }
+    // src/absorb.h:34-39 exits the process; this reports, records LastError and returns
+    if (!wfd::check_freq_q(freqQ)) return;
     wfd::setup_q_2d(P2(p), freqQ, qMin, qInterior);
 }
 

@@ -520,7 +520,9 @@ static inline bool setup_q_2d(Prop *p, float freqQ, float qMin, float qInterior)
     }
     const long ns = p->nsponge, nx = p->nx, nz = p->nz;
     std::vector<float> qprof(ns);
-    const double lqmin = std::log((double)qMin), lqmax = std::log((double)qInterior);
+    // the reference's log(qMin) on a float argument is C++'s float overload (logf), widened to
+    // double for the ramp (src/absorb.h:47-48)
+    const double lqmin = (double)std::log(qMin), lqmax = (double)std::log(qInterior);
     for (long ks = 0; ks < ns; ks++) {
         const double dk = (double)(ks) / (double)(ns - 1);
         const double lq = lqmin + dk * (lqmax - lqmin);

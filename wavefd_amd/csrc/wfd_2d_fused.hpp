@@ -156,7 +156,9 @@ __device__ __forceinline__ void fused2_body(const Fused2Args<PHYS> &A, Fused2Lds
     };
     // output operand k of plane xx: Q from the table while it is the profile (the index clamped
     // into the table for cells whose value is never used)
-    const int kzs = FST ? nz - 1 - z : min(z, nz - 1 - z);
+    // (the grid's free surface, not the tile's: every z-strip of a free-surface grid measures the
+    // distance to the bottom row only, src/absorb.h:63)
+    const int kzs = A.g.fs ? nz - 1 - z : min(z, nz - 1 - z);
     auto opnd = [&](int k, int xx, int off) -> float {
         if (k == 1 && A.qtab) {
             int ks = min(min(xx, nx - 1 - xx), kzs);

@@ -30,13 +30,9 @@
     }                                                                                                     \
     void WFD_CAT(Prefix, free)(void *p) { delete reinterpret_cast<wfd::Prop *>(p); }                      \
     void WFD_CAT(Prefix, SetupDtOmegaInvQ)(void *p, float freqQ, float qMin, float qInterior) {           \
-        /* src/absorb.h:98-103: the reference reports and terminates the process */                     \
-        if (freqQ < FLT_EPSILON) {                                                                        \
-            char msg[1000];                                                                               \
-            std::snprintf(msg, sizeof(msg), "Error -- freqQ [%f] is too small!\n", freqQ);                \
-            std::perror(msg);                                                                             \
-            std::exit(EXIT_FAILURE);                                                                      \
-        }                                                                                                 \
+        /* src/absorb.h:98-103 reports and terminates the process; this library reports the same   */ \
+        /* message, records it for LastError and returns with the handle's Q unchanged             */ \
+        if (!wfd::check_freq_q(freqQ)) return;                                                            \
         wfd::setup_q(reinterpret_cast<wfd::Prop *>(p), freqQ, qMin, qInterior);                           \
     }                                                                                                     \
     void WFD_CAT(Prefix, TimeStep)(void *p) { wfd::time_step(reinterpret_cast<wfd::Prop *>(p), false); } \

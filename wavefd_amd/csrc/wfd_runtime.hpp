@@ -395,8 +395,19 @@ static inline Prop *make_prop(Physics phys, long fs, long nthread, long nx, long
     return p;
 }
 
-// src/absorb.h:85-151.  Returns false (after the reference's message) on freqQ < FLT_EPSILON;
-// the C-ABI wrapper then exits the process as the reference does.
+// The reference rejects freqQ < FLT_EPSILON with a message and exit(EXIT_FAILURE)
+// (src/absorb.h:34-39, :98-103).  A library must not end its host process: this prints the same
+// message, records it for LastError and returns false; the caller's handle is left unchanged.
+static inline bool check_freq_q(float freqQ) {
+    if (freqQ >= FLT_EPSILON) return true;
+    char msg[1000];
+    std::snprintf(msg, sizeof(msg), "Error -- freqQ [%f] is too small!\n", freqQ);
+    std::perror(msg);
+    set_error(std::string("SetupDtOmegaInvQ: ") + msg);
+    return false;
+}
+
+// src/absorb.h:85-151
 static inline bool setup_q(Prop *p, float freqQ, float qMin, float qInterior) {
     if (!p->begin(bit(F_Q))) return false;
     if ((qMin < FLT_EPSILON) && (qInterior < FLT_EPSILON)) {

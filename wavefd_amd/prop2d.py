@@ -40,7 +40,14 @@ class _Prop2D:
         self.p = self._lib.alloc(fs, nthreads, nx, nz, nsponge, dx, dz, dt, nbx, nbz)
         if not self.p:
             raise RuntimeError("alloc failed: " + _lib.last_error(self.KIND))
+        self._lib.ClearError()
         self._lib.SetupDtOmegaInvQ(self.p, freqQ, qMin, qInterior)
+        err = _lib.last_error(self.KIND)
+        if err:  # freqQ < FLT_EPSILON (src/absorb.h:98-103): the reference exits; this raises
+            self._lib.ClearError()
+            self._lib.free(self.p)
+            self.p = None
+            raise ValueError(err.strip())
         self.shape = (nz, nx)
         self.n = nz * nx
 

@@ -81,7 +81,14 @@ class _Prop3D:
         self.p = self._lib.alloc(fs, nthreads, nx, ny, nz, nsponge, dx, dy, dz, dt, nbx, nby, nbz)
         if not self.p:
             raise RuntimeError("alloc failed: " + _lib.last_error(self.KIND))
+        self._lib.ClearError()
         self._lib.SetupDtOmegaInvQ(self.p, freqQ, qMin, qInterior)
+        err = _lib.last_error(self.KIND)
+        if err:  # freqQ < FLT_EPSILON (src/absorb.h:98-103): the reference exits; this raises
+            self._lib.ClearError()
+            self._lib.free(self.p)
+            self.p = None
+            raise ValueError(err.strip())
         self.shape = (nz, ny, nx)
         self.n = nz * ny * nx
         self.dt = dt
