@@ -44,7 +44,7 @@ def test_results_do_not_depend_on_the_configuration(kind):
     shape = (80, 40, 70)
     _, ref = _run(kind, shape, None)
     for config in (("64x16", False, 0), ("32x32", False, 0), ("64x16", False, 27), ("32x32", False, 14),
-                   ("64x16", False, 80)):
+                   ("64x16", False, 80), ("wide", False, 0), ("wide", False, 9)):
         eff, got = _run(kind, shape, config)
         assert eff["tile"] == config[0]
         if config[2]:
@@ -54,12 +54,14 @@ def test_results_do_not_depend_on_the_configuration(kind):
 
 
 @pytest.mark.parametrize("kind", ["vti", "tti"])
-def test_rounds_in_step_change_no_result(kind):
+@pytest.mark.parametrize("tile,chunk", [("64x16", 2), ("wide", 1)])
+def test_rounds_in_step_change_no_result(kind, tile, chunk):
     # enough workgroups for 20 rounds of one per CU (5120 on 256 CUs): 45 tiles per plane
     # (128 x 128, 64 x 16 tile) times 128 chunks of 2 x-planes
     shape = (256, 128, 128)
-    eff_off, ref = _run(kind, shape, ("64x16", False, 2), nsteps=2)
-    eff_on, got = _run(kind, shape, ("64x16", True, 2), nsteps=2)
+    # (wide tile: 24 tiles per plane times 256 chunks of one x-plane)
+    eff_off, ref = _run(kind, shape, (tile, False, chunk), nsteps=2)
+    eff_on, got = _run(kind, shape, (tile, True, chunk), nsteps=2)
     assert not eff_off["lockstep"]
     assert eff_on["lockstep"], eff_on
     for a, b in zip(ref, got):
@@ -69,7 +71,7 @@ def test_rounds_in_step_change_no_result(kind):
 def test_set_fused_config_rejects_bad_values():
     import wavefd_amd as w
     p = _prop("iso", (24, 20, 22))
-    for bad in ((3, 0, 0), (0, 2, 0), (0, 0, -1)):
+    for bad in ((4, 0, 0), (-1, 0, 0), (0, 2, 0), (0, 0, -1)):
         assert p._lib.SetFusedConfig(p.p, *bad) == -1
         p._lib.ClearError()
     w.free(p)

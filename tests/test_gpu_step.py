@@ -11,7 +11,7 @@ from tests.helpers import MODEL, host_view, make_fields, make_model, rel_l2, to_
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(37, 41, 45), (20, 23, 61)]
+SHAPES = [(37, 41, 45), (20, 23, 61), (19, 60, 130)]
 
 
 def _pair(oracle, kind, shape, fs, seed, nsponge=6):
@@ -55,7 +55,7 @@ def test_q_profile_matches_oracle(oracle, kind, fs):
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [False, True])
-@pytest.mark.parametrize("fused", ["64x16", "32x32", None])
+@pytest.mark.parametrize("fused", ["64x16", "32x32", "wide", None])
 @pytest.mark.parametrize("shape", SHAPES)
 def test_steps_bit_exact(oracle, kind, fs, fused, shape):
     import wavefd_amd as w
@@ -70,11 +70,28 @@ def test_steps_bit_exact(oracle, kind, fs, fused, shape):
     _compare(prop, orc, kind, shape, names)
 
 
+@pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
+@pytest.mark.parametrize("chunk", [5, 13])
+def test_wide_tile_short_chunks_bit_exact(oracle, kind, chunk):
+    # several x-chunks per grid, each padded to whole 8-step rounds of the wide kernel's loop
+    import wavefd_amd as w
+    shape = (37, 41, 70)
+    prop, orc = _pair(oracle, kind, shape, True, 5)
+    w.set_fused_config(prop, "wide", False, chunk)
+    for _ in range(3):
+        w.propagateforward(prop)
+        orc.time_step()
+    names = ["pCur", "pOld", "pSpace"] + (["mCur", "mOld", "mSpace"] if kind != "iso" else [])
+    _compare(prop, orc, kind, shape, names)
+
+
 @pytest.mark.parametrize("fs", [False, True])
-def test_vti_linear_step_bit_exact(oracle, fs):
+@pytest.mark.parametrize("tile", ["auto", "wide"])
+def test_vti_linear_step_bit_exact(oracle, fs, tile):
     import wavefd_amd as w
     shape = (30, 29, 33)
     prop, orc = _pair(oracle, "vti", shape, fs, 3)
+    w.set_fused_tile(prop, tile)
     for _ in range(4):
         w.propagateforward_linear(prop)
         orc.time_step_linear()

@@ -105,6 +105,7 @@ __device__ __forceinline__ float gld(const char *base, uint32_t off) {
 __device__ __forceinline__ void gst(float v, float *a, int xx, long nynz, uint32_t off) {
     *reinterpret_cast<float *>(reinterpret_cast<char *>(a + (long)xx * nynz) + off) = v;
 }
+
 __device__ __forceinline__ uint32_t launder(uint32_t off) {
     asm volatile("" : "+v"(off));
     return off;
@@ -567,8 +568,8 @@ __global__ void __launch_bounds__(ZW *NROW) k_fused_batch(FusedBatch<PHYS> B) {
 }
 
 // x-planes per chunk: balance whole rounds of workgroups (one per CU) against the 7-step
-// warm-up and the padding to a multiple of 4 steps that each chunk pays
-static inline int fused_cx(const Prop *p, long tiles) {
+// warm-up and the padding to a multiple of `unroll` steps that each chunk pays
+static inline int fused_cx(const Prop *p, long tiles, int unroll = 4) {
     if (p->fused_chunk > 0) return (int)std::min<long>(p->fused_chunk, p->nx);
     if (const char *e = std::getenv("WAVEFD_FUSED_CX")) {
         const long v = std::atol(e);
@@ -579,7 +580,7 @@ static inline int fused_cx(const Prop *p, long tiles) {
     double best = 1e300;
     for (long nxc = 1; nxc <= 64 && nxc <= p->nx; nxc++) {
         const long cx = (p->nx + nxc - 1) / nxc;
-        const long steps = (cx + 7 + 3) / 4 * 4;
+        const long steps = (cx + 7 + unroll - 1) / unroll * unroll;
         const long rounds = (tiles * nxc + cus - 1) / cus;
         const double cost = (double)rounds * (double)steps;
         if (cost < best * 0.999) {
@@ -593,7 +594,8 @@ static inline int fused_cx(const Prop *p, long tiles) {
 // e0, e1, e2: the epilogue's arrays (device), see Epilogue: SAVE_DP (DP, DM, -),
 // BORN_INJ (dV, DP, DM), IMAGE (g, DP, DM).  Returns the number of tiles.
 template <int PHYS, int ZW, int NROW, int EPI>
-static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float *e2, FusedArgs<PHYS> &A) {
+static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float *e2, FusedArgs<PHYS> &A,
+                              int unroll = 4, int (*cx_rule)(const Prop *, long, int, int) = nullptr) {
     constexpr int ZO = ZW - 7, YO = NROW - 7;
     const Field *f = p->f;
     A.g = p->g;
@@ -618,7 +620,7 @@ static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float 
     }
     A.nzt = (int)((p->nz + ZO - 1) / ZO);
     A.nyt = (int)((p->ny + YO - 1) / YO);
-    A.cx = fused_cx(p, (long)A.nzt * A.nyt);
+    A.cx = cx_rule ? cx_rule(p, (long)A.nzt * A.nyt, PHYS, EPI) : fused_cx(p, (long)A.nzt * A.nyt, unroll);
     // blocks of (CUs per XCD) tiles, 4 along z
     const int xcd_cus = std::max(1, p->cu_count / 8);
     A.bz = std::min(4, A.nzt);
@@ -685,6 +687,12 @@ static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float
     return true;
 }
 
+}  // namespace wfd
+
+#include "wfd_wide.hpp"
+
+namespace wfd {
+
 static inline bool fused_fits(Prop *p) {
     // plane offsets are 32-bit byte offsets
     if (p->ny * p->nz >= (1L << 29)) {
@@ -694,27 +702,32 @@ static inline bool fused_fits(Prop *p) {
     return true;
 }
 
-// tile shapes of the fused kernel (threads along z x along y; outputs (z-7) x (y-7))
-enum FusedTile { TILE_AUTO = 0, TILE_64x16 = 1, TILE_32x32 = 2 };
+// tile shapes of the fused kernel (temporaries along z x along y; outputs (z-7) x (y-7)):
+// 64 x 16 and 32 x 32 with one thread per cell (k_fused_step), 64 x 32 with four y-rows per
+// thread for Iso, 64 x 24 with three for VTI and TTI (k_wide_step)
+enum FusedTile { TILE_AUTO = 0, TILE_64x16 = 1, TILE_32x32 = 2, TILE_64x32W = 3 };
 
 static inline bool set_fused_tile(Prop *p, long tile) {
-    if (tile < TILE_AUTO || tile > TILE_32x32) {
-        set_error("SetFusedTile: tile must be 0 (automatic), 1 (64x16) or 2 (32x32)");
+    if (tile < TILE_AUTO || tile > TILE_64x32W) {
+        set_error("SetFusedTile: tile must be 0 (automatic), 1 (64x16), 2 (32x32) or 3 (wide: 64x32 Iso, 64x24 VTI/TTI)");
         return false;
     }
     p->fused_tile = (int)tile;
     return true;
 }
 
-// automatic choice (measured on MI355X, 800^3 TTI, 512^3 VTI, 256^3 Iso): the square tile
-// recomputes fewer halo temporaries per output (1024 / 625 threads per output against
-// 1024 / 504) but requests more cache lines per output (its 32-float rows straddle two 128-byte
-// lines); Iso, with one material field, gains (-13 %), VTI and TTI lose (+7 %, +11 %)
+// automatic choice (measured on MI355X, 800^3 TTI, 512^3 VTI, 256^3 Iso).  Plain VTI and TTI
+// steps: the wide tile (3 y-rows per thread, 57 x 17 outputs), whose straight-line rows give each
+// wave the instruction-level parallelism the 64 x 16 tile gets from more waves (VTI 512^3 2.34
+// against 2.72 ms, TTI 800^3 12.39 against 12.69 ms).  Plain Iso step: the square 32 x 32 tile,
+// which recomputes fewer halo temporaries per output than 64 x 16 (1024 / 625 threads per output
+// against 1024 / 504; 0.168 against 0.223 ms at 256^3; the wide tile 0.180 ms)
 static inline int fused_tile_of(const Prop *p, int epi) {
     if (p->fused_tile != TILE_AUTO) return p->fused_tile;
-    // the Born epilogues read 1 to 3 more fields per output: the wide tile wins there (Iso Born
-    // passes on 200^3..600^3 windows: 6798 against 6416 shots per hour)
-    return p->phys == ISO && epi == EPI_NONE ? TILE_32x32 : TILE_64x16;
+    if (epi == EPI_NONE) return p->phys == ISO ? TILE_32x32 : TILE_64x32W;
+    // the Born epilogues read 1 to 3 more fields per output: the 64 x 16 tile (Iso Born passes on
+    // 200^3..600^3 windows: 6798 against 6416 shots per hour for the square one)
+    return TILE_64x16;
 }
 
 // the fused step's launch configuration (tile, rounds in step, x-chunk length); results do not
@@ -738,9 +751,10 @@ static inline bool get_fused_config(const Prop *p, long *out) {
         return false;
     }
     const int tile = fused_tile_of(p, EPI_NONE);
-    const int zo = (tile == TILE_32x32 ? 32 : 64) - 7, yo = (tile == TILE_32x32 ? 32 : 16) - 7;
+    const int zo = (tile == TILE_32x32 ? 32 : 64) - 7;
+    const int yo = (tile == TILE_32x32 ? 32 : (tile == TILE_64x32W ? wide_rows(p->phys, EPI_NONE) : 16)) - 7;
     const long tiles = ((p->nz + zo - 1) / zo) * ((p->ny + yo - 1) / yo);
-    const long cx = fused_cx(p, tiles);
+    const long cx = tile == TILE_64x32W ? wide_cx(p, tiles, p->phys, EPI_NONE) : fused_cx(p, tiles, 4);
     const long ntiles = tiles * ((p->nx + cx - 1) / cx);
     out[0] = tile;
     out[1] = fused_lockstep_of(p, p->phys, EPI_NONE, ntiles) ? 1 : 0;
@@ -751,7 +765,9 @@ static inline bool get_fused_config(const Prop *p, long *out) {
 
 template <int PHYS, int EPI>
 static inline bool launch_fused_tile(Prop *p, bool linear, float *e0, float *e1, float *e2) {
-    if (fused_tile_of(p, EPI) == TILE_32x32) return launch_fused<PHYS, 32, 32, EPI>(p, linear, e0, e1, e2);
+    const int tile = fused_tile_of(p, EPI);
+    if (tile == TILE_64x32W) return launch_wide<PHYS, EPI>(p, linear, e0, e1, e2);
+    if (tile == TILE_32x32) return launch_fused<PHYS, 32, 32, EPI>(p, linear, e0, e1, e2);
     return launch_fused<PHYS, 64, 16, EPI>(p, linear, e0, e1, e2);
 }
 
@@ -805,6 +821,7 @@ static inline bool launch_fused_batch(Prop **ps, int n, float *const *e0, float 
 
 template <int PHYS, int EPI>
 static inline bool launch_fused_batch_tile(Prop **ps, int n, float *const *e0, float *const *e1, float *const *e2) {
+    // (the several-grid launch has the one-thread-per-cell tiles only; the wide tile runs as 64 x 16)
     if (fused_tile_of(ps[0], EPI) == TILE_32x32) return launch_fused_batch<PHYS, 32, 32, EPI>(ps, n, e0, e1, e2);
     return launch_fused_batch<PHYS, 64, 16, EPI>(ps, n, e0, e1, e2);
 }

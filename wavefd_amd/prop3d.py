@@ -222,7 +222,7 @@ def set_fused_step(prop, fused=True):
     prop._lib.SetFusedStep(prop.p, 1 if fused else 0)
 
 
-FUSED_TILES = {"auto": 0, "64x16": 1, "32x32": 2}
+FUSED_TILES = {"auto": 0, "64x16": 1, "32x32": 2, "wide": 3}
 
 
 def set_fused_tile(prop, tile="auto"):

@@ -70,7 +70,7 @@ def fused_config(prop):
     out = (ctypes.c_long * 4)()
     if prop._lib.GetFusedConfig(prop.p, out) != 0:
         prop._check()
-    tile = {1: "64x16", 2: "32x32"}[out[0]]
+    tile = {1: "64x16", 2: "32x32", 3: "wide"}[out[0]]
     return {"tile": tile, "lockstep": bool(out[1]), "chunk": int(out[2]), "workgroups": int(out[3])}
 
 
@@ -87,7 +87,7 @@ def _candidates(prop):
     x-chunk lengths that split nx into 1, 2, 3, 4, 6 or 8 chunks besides the automatic one."""
     nx = prop.shape[2]
     seen, out = set(), []
-    for tile in ("64x16", "32x32"):
+    for tile in ("64x16", "32x32", "wide"):
         for lockstep in (False, True):
             for nxc in (0, 1, 2, 3, 4, 6, 8):
                 chunk = 0 if nxc == 0 else -(-nx // nxc)
