@@ -33,6 +33,14 @@ struct SourcePoint64 {
 /* src/spacetime.cc:144-162 -- field[iu-1] += c * data[(ir-1)*nt + it-1] */
 void injectdata_float(float *field, float *data, size_t it, size_t nt, struct SourcePoint32 **partitions,
                       size_t npartitions, long *npoints_per_partition, size_t nthreads);
+/* No counterpart in the reference.  injectdata_float for callers that cannot allocate (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc): the same accumulation, bit
+ * for bit, without any device or host allocation.  field and data must be device memory of the
+ * current device; the points are staged in a fixed device array of at most 1,048,576 points in at
+ * most 4,096 partitions (larger inputs are rejected, see spacetime_LastError).  Every call on a GPU
+ * uses the same array: calls that overlap on one GPU race. */
+void injectdata_float_noalloc(float *field, float *data, size_t it, size_t nt, struct SourcePoint32 **partitions,
+                              size_t npartitions, long *npoints_per_partition, size_t nthreads);
 /* src/spacetime.cc:164-182 */
 void injectdata_double(double *field, double *data, size_t it, size_t nt, struct SourcePoint64 **partitions,
                        size_t npartitions, long *npoints_per_partition, size_t nthreads);

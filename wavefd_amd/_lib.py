@@ -95,6 +95,7 @@ for _a in ("SinTheta", "CosTheta", "SinPhi", "CosPhi"):
 
 SPACETIME = {
     "injectdata_float": (None, [_P, _P, _S, _S, _P, _S, _P, _S]),
+    "injectdata_float_noalloc": (None, [_P, _P, _S, _S, _P, _S, _P, _S]),
     "injectdata_double": (None, [_P, _P, _S, _S, _P, _S, _P, _S]),
     "extractdata_float": (None, [_P, _P, _P, _S, _P, _S, _S, _S]),
     "extractdata_double": (None, [_P, _P, _P, _S, _P, _S, _S, _S]),

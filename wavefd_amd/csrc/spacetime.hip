@@ -138,6 +138,159 @@ void device_scatter_gather(bool inject, T *field, T *data, size_t it, size_t nt,
     if (ddata) hipFree(ddata);
 }
 
+// ---- injection without allocation (injectdata_float_noalloc) ---------------------------
+// For callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+// harnesses that forbid hipMalloc).  The partitions' points are staged in a file-scope device
+// array, `scratch`, of NOALLOC_MAX_POINTS points and NOALLOC_MAX_PARTS + 1 partition offsets,
+// which each GPU the library runs on holds until the process ends; no hipMalloc, hipFree or
+// host heap allocation happens per call.  Every call on a GPU shares that GPU's array: calls that
+// overlap on one GPU (from several host threads) race.  Nothing in it needs clearing: a call
+// writes every entry it reads.
+constexpr long NOALLOC_MAX_POINTS = 1L << 20;  // 24 MiB of SourcePoint32
+constexpr long NOALLOC_MAX_PARTS = 4096;
+
+struct NoallocScratch {
+    SourcePoint32 pts[NOALLOC_MAX_POINTS];
+    long off[NOALLOC_MAX_PARTS + 1];  // partition p holds points [off[p], off[p+1])
+};
+static __device__ __align__(256) NoallocScratch scratch;
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+NoallocScratch *g_scratch_ptr[NOALLOC_MAX_DEVICES];  // hipGetSymbolAddress of `scratch`, per device
+
+// the device address of this device's `scratch` (looked up once per device, with it current)
+NoallocScratch *scratch_of(int dev) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return nullptr;
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            report(std::string("injectdata_float_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = reinterpret_cast<NoallocScratch *>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+
+// one thread per point; a point that starts a run of equal iu inside its partition adds the run's
+// contributions in partition order (runs of one key are disjoint when partitions are sorted by
+// iu, as src/spacetime.jl:1049-1129 builds them and the host checks before this launch)
+__global__ void k_inject_runs(float *__restrict__ field, const float *__restrict__ data, long it, long nt,
+                              const NoallocScratch *__restrict__ s, long nparts, long npts) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    long lo = 0, hi = nparts;  // the partition p with off[p] <= i < off[p+1]
+    while (hi - lo > 1) {
+        const long mid = (lo + hi) / 2;
+        if (s->off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    const long end = s->off[lo + 1];
+    const long iu = s->pts[i].iu;
+    if (i > s->off[lo] && s->pts[i - 1].iu == iu) return;
+    float acc = field[iu - 1];
+    for (long j = i; j < end && s->pts[j].iu == iu; j++) acc += s->pts[j].c * data[(s->pts[j].ir - 1) * nt + it - 1];
+    field[iu - 1] = acc;
+}
+
+// one thread per partition, its points in order: the reference's loop (src/spacetime.cc:144-162)
+// for partitions not sorted by iu
+__global__ void k_inject_parts(float *__restrict__ field, const float *__restrict__ data, long it, long nt,
+                               const NoallocScratch *__restrict__ s, long nparts) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nparts) return;
+    for (long j = s->off[p]; j < s->off[p + 1]; j++) {
+        const SourcePoint32 q = s->pts[j];
+        field[q.iu - 1] += q.c * data[(q.ir - 1) * nt + it - 1];
+    }
+}
+
+// number of floats from ptr to the end of its device allocation, or -1
+long device_extent(const void *ptr) {
+    hipDeviceptr_t base = nullptr;
+    size_t span = 0;
+    if (hipMemGetAddressRange(&base, &span, (hipDeviceptr_t)ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    return (long)((span - (size_t)((const char *)ptr - (const char *)base)) / sizeof(float));
+}
+
+void inject_noalloc(float *field, float *data, size_t it, size_t nt, SourcePoint32 **parts, size_t nparts,
+                    const long *npts) {
+    const char *who = "injectdata_float_noalloc: ";
+    if (nparts > (size_t)NOALLOC_MAX_PARTS) {
+        report(std::string(who) + "more than " + std::to_string(NOALLOC_MAX_PARTS) + " partitions (" +
+               std::to_string(nparts) + ")");
+        return;
+    }
+    long total = 0;
+    for (size_t i = 0; i < nparts; i++) {
+        if (npts[i] < 0) {
+            report(std::string(who) + "negative point count");
+            return;
+        }
+        total += npts[i];
+        if (total > NOALLOC_MAX_POINTS) {
+            report(std::string(who) + "more than " + std::to_string(NOALLOC_MAX_POINTS) + " points");
+            return;
+        }
+    }
+    if (total == 0) return;
+    hipPointerAttribute_t fa, da;
+    std::memset(&fa, 0, sizeof(fa));
+    std::memset(&da, 0, sizeof(da));
+    int cur = -1;
+    if (hipPointerGetAttributes(&fa, field) != hipSuccess || hipPointerGetAttributes(&da, data) != hipSuccess ||
+        fa.type != hipMemoryTypeDevice || da.type != hipMemoryTypeDevice || hipGetDevice(&cur) != hipSuccess ||
+        fa.device != cur || da.device != cur) {
+        (void)hipGetLastError();
+        report(std::string(who) + "field and data must be device memory of the current device");
+        return;
+    }
+    const long nfield = device_extent(field), ndata = device_extent(data);
+    if (nfield < 0 || ndata < 0 || it < 1 || it > nt) {
+        report(std::string(who) + "cannot size the device arrays, or it outside 1..nt");
+        return;
+    }
+    // every access in bounds; runs of equal iu inside each partition contiguous (sorted by iu)
+    bool sorted = true;
+    for (size_t p = 0; p < nparts; p++)
+        for (long j = 0; j < npts[p]; j++) {
+            const SourcePoint32 &q = parts[p][j];
+            if (q.iu < 1 || q.iu > nfield || q.ir < 1 || (q.ir - 1) * (long)nt + (long)it > ndata) {
+                report(std::string(who) + "point outside the device field or data (iu=" + std::to_string(q.iu) +
+                       ", ir=" + std::to_string(q.ir) + ")");
+                return;
+            }
+            if (j > 0 && q.iu < parts[p][j - 1].iu) sorted = false;
+        }
+    NoallocScratch *S = scratch_of(cur);
+    if (!S) return;
+    // stage the points and the partition offsets (synchronous copies on the null stream, ordered
+    // after the previous call's kernel, which read this same array)
+    long hoff[NOALLOC_MAX_PARTS + 1];  // (on the stack)
+    hoff[0] = 0;
+    hipError_t e = hipSuccess;
+    for (size_t p = 0; p < nparts; p++) {
+        if (e == hipSuccess && npts[p] > 0)
+            e = hipMemcpy(&S->pts[hoff[p]], parts[p], sizeof(SourcePoint32) * npts[p], hipMemcpyHostToDevice);
+        hoff[p + 1] = hoff[p] + npts[p];
+    }
+    if (e == hipSuccess) e = hipMemcpy(&S->off[0], hoff, sizeof(long) * (nparts + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        if (sorted)
+            k_inject_runs<<<(unsigned)((total + 255) / 256), 256>>>(field, data, (long)it, (long)nt, S, (long)nparts,
+                                                                    total);
+        else
+            k_inject_parts<<<(unsigned)((nparts + 63) / 64), 64>>>(field, data, (long)it, (long)nt, S, (long)nparts);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) report(std::string(who) + hipGetErrorString(e));
+}
+
 // ---- host paths: src/spacetime.cc:144-204 ---------------------------------------------
 
 template <class P, class T>
@@ -223,6 +376,12 @@ void injectdata_float(float *field, float *data, size_t it, size_t nt, SourcePoi
                                                     npoints_per_partition);
     else
         host_inject(field, data, it, nt, partitions, npartitions, npoints_per_partition, nthreads);
+}
+
+void injectdata_float_noalloc(float *field, float *data, size_t it, size_t nt, SourcePoint32 **partitions,
+                              size_t npartitions, long *npoints_per_partition, size_t nthreads) {
+    (void)nthreads;
+    inject_noalloc(field, data, it, nt, partitions, npartitions, npoints_per_partition);
 }
 
 void injectdata_double(double *field, double *data, size_t it, size_t nt, SourcePoint64 **partitions,
