@@ -82,3 +82,54 @@ def ulp_diff(a, b):
     ai = a.view(np.int32).astype(np.int64)
     bi = b.view(np.int32).astype(np.int64)
     return int(np.abs(ai - bi).max()) if a.size else 0
+
+
+def smooth_noise_big(shape, cells, seed):
+    """Seeded smooth field in [-1, 1] on a large (nx, ny, nz) grid: U(-1, 1) noise on a coarse grid
+    of one node per `cells` cells, interpolated trilinearly (three matrix products, float32), scaled
+    to max |x| = 1.  A cheap stand-in for the Gaussian smoothing of the config models at sizes where
+    filtering the full grid on the host would dominate a test."""
+    rng = np.random.default_rng(seed)
+    coarse_shape = [max(2, int(np.ceil(n / cells)) + 1) for n in shape]
+    c = rng.uniform(-1, 1, coarse_shape).astype(F32)
+
+    def interp(n, m):
+        # (n, m) matrix of linear-interpolation weights from m coarse nodes to n points
+        t = np.arange(n) * (m - 1) / max(n - 1, 1)
+        i0 = np.minimum(np.floor(t).astype(int), m - 2)
+        w1 = (t - i0).astype(F32)
+        a = np.zeros((n, m), F32)
+        a[np.arange(n), i0] = 1 - w1
+        a[np.arange(n), i0 + 1] += w1
+        return a
+
+    ax, ay, az = (interp(n, m) for n, m in zip(shape, coarse_shape))
+    x = np.tensordot(ax, c, axes=(1, 0))                      # (nx, my, mz)
+    x = np.einsum("jb,ibc->ijc", ay, x, optimize=True)         # (nx, ny, mz)
+    out = (x.reshape(-1, x.shape[2]) @ az.T).reshape(shape)   # (nx, ny, nz)
+    out /= np.abs(out).max()
+    return out.astype(F32)
+
+
+def config_model(physics, shape, d, dt, seed):
+    """The configs' model construction in oracle layout (nx, ny, nz): v(z) rising linearly from
+    1500 m/s at the top to 4500 m/s at the bottom times 1 +- 5 % smooth noise, eps in [0, 0.3] and
+    eta in [0, 0.15] (smooth), f = 0.85, velocity capped so that v sqrt(1 + 2 eps) dt / d <= 0.42,
+    density from Gardner's relation clamped to [1, 2.5] g/cc, b = 1 / rho."""
+    nx, ny, nz = shape
+    z = np.arange(nz, dtype=np.float64) * d
+    vz = (1500.0 + 3000.0 * z / ((nz - 1) * d)).astype(F32)
+    v = vz[None, None, :] * (1 + 0.05 * smooth_noise_big(shape, 15, seed))
+    m = {}
+    cap = F32(0.42 * d / dt)
+    if physics != "iso":
+        m["eps"] = (0.15 + 0.15 * smooth_noise_big(shape, 15, seed + 100)).astype(F32)
+        m["eta"] = (0.075 + 0.075 * smooth_noise_big(shape, 15, seed + 200)).astype(F32)
+        m["f"] = np.full(shape, 0.85, F32)
+        v = np.minimum(v, cap / np.sqrt(1 + 2 * m["eps"]))
+    else:
+        v = np.minimum(v, cap)
+    m["v"] = v.astype(F32)
+    rho = np.clip(0.31 * m["v"] ** 0.25, 1.0, 2.5)
+    m["b"] = (1.0 / rho).astype(F32)
+    return m

@@ -259,3 +259,47 @@ def test_dequantize_with_known_shape_equals_header_read():
     assert np.array_equal(_get(a.ptr, n), _get(b.ptr, n))
     for q in (xb, cb, a, b):
         q.free()
+
+
+@pytest.mark.parametrize("kind", ["iso", "vti"])
+def test_wave_predicted_store_against_exact(kind):
+    """born.py::_WaveStore: frames predicted by one step of a propagator with the window's model
+    plus the source's image, residuals quantised; the Born image within 1e-4 of exact storage
+    (an error each frame carries is coded away with the next frame's residual), the stored bytes
+    those of 2 keyframes and k - 2 residual codes per segment."""
+    from wavefd_amd import born, ginsu, lcodec, spacetime
+    shape = (64, 56, 60)
+    nx, ny, nz = shape
+    nt, k = 160, 40
+    m = make_model(kind, shape, 51)
+    dev = {key: _dev(m[key]) for key in born.MODEL_FIELDS[kind]}
+    dv = smooth(np.random.default_rng(52), shape, 40.0, 2).astype(F32)
+    dvb = _dev(dv)
+    wav = spacetime.causal_ricker(15.0, 1e-3 * np.arange(nt))
+    sh = born.BornShot((45.0, 280.0, 300.0), np.full(9, 35.0), np.linspace(150, 420, 9), np.linspace(160, 450, 9), wav)
+    win = ginsu.Window((nz, ny, nx), (0, 0, 0), (nz, ny, nx))
+    model = {key: b.ptr for key, b in dev.items()}
+    imgs, stored = {}, None
+    for mode in ("exact", "wave"):
+        img = ginsu.DeviceBuffer(dv.nbytes)
+        img.zero()
+        r = born.born_shot(kind, win, (10.0, 10.0, 10.0), 1e-3, sh, model, dvb.ptr, img.ptr, nsponge=10,
+                           predictive=(k, "wave", 16) if mode == "wave" else None)
+        imgs[mode] = img.download(np.zeros(shape, F32)).astype(np.float64)
+        if mode == "wave":
+            stored = r["stored_bytes"]
+        img.free()
+    for b in list(dev.values()) + [dvb]:
+        b.free()
+    rel = np.linalg.norm(imgs["wave"] - imgs["exact"]) / np.linalg.norm(imgs["exact"])
+    nf = 1 if kind == "iso" else 2
+    import ctypes
+    from wavefd_amd import _lib
+    cod = _lib.load("codec")
+    kwords = cod.wavefd_codec_words((ctypes.c_long * 3)(nz, ny, nx), (ctypes.c_long * 3)(8, 8, 8), 16)
+    nkey = 2 * (nt // k)
+    want = 4 * (kwords * nkey + lcodec.code_words((nz, ny, nx)) * (nt - nkey)) * nf
+    print(f"wave-predicted store {kind}: image rel err {rel:.3e}, {4 * nx * ny * nz * nt * nf / stored:.2f}x",
+          flush=True)
+    assert stored == want
+    assert np.abs(imgs["exact"]).max() > 0 and rel <= 1e-4

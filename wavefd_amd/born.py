@@ -422,6 +422,183 @@ class _PredictiveStore:
             b.free()
 
 
+class _WaveStore:
+    """DP_n (DM_n) kept by closed-loop prediction with the propagator itself.
+
+    DP_n = (v^2 / b) L(p_n) is linear in p_n, and p_n = step(p_{n-1}, p_{n-2}) + w_n S (the
+    registered source taps S scaled by the wavelet), so where Q is locally constant DP obeys the
+    same leapfrog as the wavefield plus the source's image: DP_{n+1} = dt^2 (v^2/b) L(DP_n)
+    - g (DP_n - DP_{n-1}) - DP_{n-1} + 2 DP_n + w_n G with G = (v^2/b) L(S) -- one TimeStep
+    applied to (DP_n, DP_{n-1}) (for VTI to the pair (DP, DM), the coupled system being linear in
+    (p, m) too) and one scaled add.  The store runs that step on a second propagator with the
+    window's model and Q, whose pCur/pOld (mCur/mOld) buffers hold the last two reconstructions,
+    forms G once per shot with the same propagator, and codes only the residual of each frame
+    against the prediction (the 11-level block quantiser, 3.59 bits per value).  The prediction
+    misses only the sponge's term (where Q varies), and the encoder predicts from its own
+    reconstructions: an error that a frame carries is propagated into the next prediction and
+    coded away with that frame's residual, so errors shrink from frame to frame instead of
+    persisting.  Frames fall into segments of k; the
+    first two frames of a segment are keyframes (the block codec, `keybits` bits).  Decoding a
+    segment repeats the encoder's kernel calls (keyframe decodes, steps, dequantisation, the same
+    combination), so its reconstructions are the encoder's bit for bit.
+
+    Stored bytes: 2 keyframes and k - 2 residual codes per segment and wavefield (k = 96 with
+    16-bit keyframes: 8.3x fewer than float32; measured on VTI 512^3, nt = 400: Born image within
+    6.5e-6 of exact storage at k = 48, 1.3e-3 with 8-bit keyframes, whose own error then
+    dominates); working fields: the segment buffer (k per wavefield), the frame the step writes,
+    the residual, the source image G and the second propagator."""
+
+    def __init__(self, run, k=96, keybits=16):
+        from . import lcodec, _lib
+        if int(k) < 3 or int(keybits) not in (8, 16):
+            raise ValueError("wave-predicted store: segments of at least 3 frames, 8- or 16-bit keyframes")
+        self.run, self.lc, self.k, self.keybits = run, lcodec, int(k), int(keybits)
+        self.shape, self.nt, self.nf = run.window.size, run.nt, run.nf
+        self.ncell = ncell = run.ncell
+        self.cod = _lib.load("codec")
+        nz, ny, nx = self.shape
+        self.n3 = (ctypes.c_long * 3)(nz, ny, nx)
+        self.b3 = (ctypes.c_long * 3)(8, 8, 8)
+        self.kwords = int(self.cod.wavefd_codec_words(self.n3, self.b3, self.keybits))
+        self.words = lcodec.code_words(self.shape)
+        self.nseg = (self.nt + self.k - 1) // self.k
+        self.nkey = sum(min(2, self.nt - j * self.k) for j in range(self.nseg))  # keyframes per wavefield
+        self.keys = ginsu.DeviceBuffer(4 * self.kwords * self.nkey * self.nf)
+        self.codes = ginsu.DeviceBuffer(max(4, 4 * self.words * (self.nt - self.nkey) * self.nf))
+        self.x = [ginsu.DeviceBuffer(4 * ncell) for _ in range(self.nf)]
+        self.tmp = ginsu.DeviceBuffer(4 * ncell)
+        self.seg = ginsu.DeviceBuffer(4 * ncell * min(self.k, self.nt) * self.nf)
+        self.cached = None
+        # the predictor: a propagator with the window's model; Q needs no copy (the same
+        # construction parameters give the same profile)
+        p2 = run.new_prop()
+        for name in MODEL_FIELDS[run.physics]:
+            ginsu.copy(prop3d.device_ptr(p2, name), prop3d.device_ptr(run.prop, name), 4 * ncell,
+                       p2._lib.GetStream(p2.p))
+        prop3d.synchronize(p2)
+        self.p2 = p2
+        self.cur = ("pCur", "mCur")
+        self.old = ("pOld", "mOld")
+        # G = (v^2/b) L(S): the unit taps in the current field(s), one step, the scaled spatial part
+        self.wavelet = np.asarray(run.shot.wavelet, F32).reshape(-1)
+        self.g = [ginsu.DeviceBuffer(4 * ncell) for _ in range(self.nf)]
+        names = ("pCur", "pOld") + (("mCur", "mOld") if self.nf == 2 else ())
+        for name in names:
+            ginsu.zero(self._ptr(name), 4 * ncell, None)
+        unit = np.ones((1, 1), F32)
+        for kind in range(self.nf):
+            spacetime.injectdata(self._ptr(self.cur[kind]), [run.src_pts], unit, 1)
+        lib = p2._lib
+        lib.TimeStep(p2.p)
+        lib.ScaleSpatialDerivatives(p2.p)
+        p2._check()
+        prop3d.synchronize(p2)
+        for kind, name in enumerate(("pSpace", "mSpace")[:self.nf]):
+            ginsu.copy(self.g[kind].ptr, self._ptr(name), 4 * ncell, None)
+        ginsu.sync(None)
+
+    def stored_bytes(self):
+        return 4 * (self.kwords * self.nkey + self.words * (self.nt - self.nkey)) * self.nf
+
+    def _key(self, kind, n):  # keyframe of frame n (the first or second of its segment)
+        j, m = divmod(n - 1, self.k)
+        i = 2 * j + m
+        return self.keys.ptr + 4 * self.kwords * (i * self.nf + kind)
+
+    def _code(self, kind, n):
+        j, m = divmod(n - 1, self.k)
+        i = n - 1 - 2 * (j + 1)  # residual frames before this one
+        return self.codes.ptr + 4 * self.words * (i * self.nf + kind)
+
+    def _slot(self, kind, n):
+        m = (n - 1) % self.k
+        return self.seg.ptr + 4 * self.ncell * (m * self.nf + kind)
+
+    def _ptr(self, name):
+        return prop3d.device_ptr(self.p2, name)
+
+    def _keyframe(self, kind, n, src, stream):
+        """code frame n of `src` as a keyframe (src None: only decode it) into the predictor's
+        current field, after moving the current one to the old one (second keyframe)"""
+        ginsu.sync(stream)  # the block codec works on the null stream
+        prop3d.synchronize(self.p2)
+        cur = self._ptr(self.cur[kind])
+        if (n - 1) % self.k == 1:
+            ginsu.copy(self._ptr(self.old[kind]), cur, 4 * self.ncell, None)
+        if src is not None and self.cod.wavefd_codec_compress(src, self.n3, self.b3, self.keybits,
+                                                               self._key(kind, n)) < 0:
+            raise RuntimeError(_lib_error("codec"))
+        if self.cod.wavefd_codec_decompress(self._key(kind, n), self.kwords, cur) != 0:
+            raise RuntimeError(_lib_error("codec"))
+        ginsu.sync(None)
+
+    def _predict(self, n, stream):
+        """the prediction of frame n: one step of the predictor, (cur, old) <- (step, cur), then
+        cur += w G with the wavelet sample injected after the step that wrote frame n - 1"""
+        ginsu.sync(stream)
+        lib = self.p2._lib
+        if self.nf == 2:
+            lib.TimeStepLinear(self.p2.p)
+        else:
+            lib.TimeStep(self.p2.p)
+        self.p2._check()
+        prop3d.synchronize(self.p2)
+        wn = float(self.wavelet[n - 2])
+        for kind in range(self.nf):
+            cur = self._ptr(self.cur[kind])
+            self.lc.combine(self.ncell, 1.0, cur, cur, wn, self.g[kind].ptr, stream=stream)
+
+    def begin(self, n):
+        pass
+
+    def out(self, kind, n):  # where the step writes DP_n
+        return self.x[kind].ptr
+
+    def commit(self, kind, n, stream):
+        lc, nc = self.lc, self.ncell
+        m = (n - 1) % self.k
+        if m < 2:
+            self._keyframe(kind, n, self.x[kind].ptr, stream)
+            return
+        if kind == 0:
+            self._predict(n, stream)
+        pred = self._ptr(self.cur[kind])
+        lc.combine(nc, 1.0, self.x[kind].ptr, self.tmp.ptr, -1.0, pred, stream=stream)  # residual
+        lc.encode(self.tmp.ptr, self.shape, self._code(kind, n), stream)
+        lc.dequantize(self._code(kind, n), self.tmp.ptr, stream, self.shape)
+        lc.combine(nc, 1.0, pred, pred, 1.0, self.tmp.ptr, stream=stream)  # the reconstruction
+        if kind == self.nf - 1:
+            ginsu.sync(stream)
+
+    def _decode_segment(self, j, stream):
+        n0 = j * self.k + 1
+        for n in range(n0, min(n0 + self.k, self.nt + 1)):
+            m = n - n0
+            if m >= 2:
+                self._predict(n, stream)
+            for kind in range(self.nf):
+                cur = self._ptr(self.cur[kind])
+                if m < 2:
+                    self._keyframe(kind, n, None, stream)
+                else:
+                    self.lc.dequantize(self._code(kind, n), self.tmp.ptr, stream, self.shape)
+                    self.lc.combine(self.ncell, 1.0, cur, cur, 1.0, self.tmp.ptr, stream=stream)
+                ginsu.copy(self._slot(kind, n), cur, 4 * self.ncell, stream)
+            ginsu.sync(stream)
+        self.cached = j
+
+    def get(self, kind, n, stream):
+        j = (n - 1) // self.k
+        if self.cached != j:
+            self._decode_segment(j, stream)
+        return self._slot(kind, n)
+
+    def free(self):
+        for b in [self.keys, self.codes, self.tmp, self.seg] + self.x + self.g:
+            b.free()
+        prop3d.free(self.p2)
+
+
 def _lib_error(name):
     from . import _lib
     return _lib.last_error(name)
@@ -483,6 +660,8 @@ class _ShotRun:
         self.born_data = None
         if checkpoint is not None:
             self.store = _CheckpointStore(self, checkpoint, fused)
+        elif isinstance(predictive, tuple) and len(predictive) > 1 and predictive[1] == "wave":
+            self.store = _WaveStore(self, predictive[0], *predictive[2:])
         elif predictive is not None:
             self.store = _PredictiveStore(window.size, nt, nf, *(predictive if isinstance(predictive, tuple)
                                                                   else (predictive,)),
