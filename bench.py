@@ -11,6 +11,10 @@ Mcells/s = nx*ny*nz*steps / t / 1e6, the reference's definition (benchmark/mcell
 Roofline: algorithmic HBM bytes of the fused step (TTI: 14 float reads + 4 float writes =
 72 B/cell) / measured kernel time, against MI355X's 8 TB/s.
 
+The default line also carries the Iso 256^3 and VTI 512^3 steps ("extra"), Born shots per hour of
+the fused path against the reference's separate calls on 24 ginsu windows ("born"), and the Born
+image with compressed snapshots against exact storage ("born.snapshot_compression").
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--physics tti|vti|iso] [--n N]
 """
 import argparse
@@ -438,6 +442,8 @@ def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000), fused=Tr
     wall = time.perf_counter() - t1
     img = image.download(np.zeros((nx, ny, nz), np.float32))
     finite = bool(np.isfinite(img).all()) and float(np.abs(img).max()) > 0
+    import hashlib
+    image_sha = hashlib.sha256(img.tobytes()).hexdigest()[:16]
     del img
     cells = sum(w.ncells for w in windows)
     if snap is not None:
@@ -445,6 +451,7 @@ def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000), fused=Tr
     for b in list(bufs.values()) + [dvb, image]:
         b.free()
     return {"physics": physics, "shots": nshots, "nt": nt, "big": list(big), "wall_s": round(wall, 3),
+            "image_sha": image_sha,
             "shots_per_hour": round(nshots * 3600.0 / wall, 1),
             "mcell_steps_per_s": round(3 * nt * cells / wall / 1e6, 1), "finite": finite,
             "setup_s": round(setup_s, 2), "window_cells_total": cells,
@@ -463,6 +470,64 @@ def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000), fused=Tr
 
 # the predictive snapshot store's settings: frames per segment, extrapolation order, keyframe bits
 PREDICTIVE = (48, 3, 16)
+
+
+def run_store(physics="vti", n=512, nt=400, seed=5, k=96):
+    """The Born gradient with compressed snapshots judged against exact storage: one shot on an
+    n^3 VTI model (the configs' construction) at 12.5 m, dt 1 ms, nt steps, a +-2 % smoothed dv;
+    the image with the wave-predicted store (born.py::_WaveStore, segments of k frames, 16-bit
+    keyframes) against the image with exact snapshots (the checkpointed store, which gives the exact
+    store's image bit for bit).  Returns the stored-byte ratio against float32, the image's relative
+    L2 error and both wall times."""
+    import torch
+    from wavefd_amd import born, ginsu
+
+    dev = torch.device("cuda:0")
+    d, dt = 12.5, 1e-3
+    hip = _hip()
+    m = build_model(torch, physics, n, d, dt, seed, dev)
+    bufs = {}
+    for name in born.MODEL_FIELDS[physics]:
+        bufs[name] = ginsu.DeviceBuffer(n ** 3 * 4)
+        torch.cuda.synchronize()
+        _copy_d2d(hip, bufs[name].ptr, m[name].contiguous())
+    dvt = (0.02 * m["v"] * _smooth_noise(torch, (n, n, n), 15, seed + 200, dev)).contiguous()
+    torch.cuda.synchronize()
+    dvb = ginsu.DeviceBuffer(n ** 3 * 4)
+    _copy_d2d(hip, dvb.ptr, dvt)
+    del m, dvt
+    torch.cuda.empty_cache()
+    win = ginsu.Window((n, n, n), (0, 0, 0), (n, n, n))
+    ry, rx = np.meshgrid(np.linspace(30 * d, (n - 30) * d, 16), np.linspace(30 * d, (n - 30) * d, 16), indexing="ij")
+    wav = np.asarray(born.spacetime.causal_ricker(15.0, dt * np.arange(nt)), np.float32)
+    shot = born.BornShot((20.0, n * d / 2, n * d / 2), np.full(ry.size, 20.0), ry.ravel(), rx.ravel(), wav)
+    fields = {k_: b.ptr for k_, b in bufs.items()}
+    imgs, secs, stored = {}, {}, None
+    for mode in ("exact", "wave"):
+        img = ginsu.DeviceBuffer(n ** 3 * 4)
+        img.zero()
+        t0 = time.perf_counter()
+        r = born.born_shot(physics, win, (d, d, d), dt, shot, fields, dvb.ptr, img.ptr, nsponge=40,
+                           keep_born_data=False, checkpoint=20 if mode == "exact" else None,
+                           predictive=(k, "wave", 16) if mode == "wave" else None)
+        img.sync()
+        secs[mode] = time.perf_counter() - t0
+        if mode == "wave":
+            stored = r["stored_bytes"]
+        imgs[mode] = img.download(np.zeros((n, n, n), np.float32)).astype(np.float64)
+        img.free()
+    for b in list(bufs.values()) + [dvb]:
+        b.free()
+    nf = 1 if physics == "iso" else 2
+    rel = float(np.linalg.norm(imgs["wave"] - imgs["exact"]) / np.linalg.norm(imgs["exact"]))
+    return {"workload": "%s %d^3 at 12.5 m, one Born shot (forward, Born forward, adjoint imaging), nt %d"
+                        % (physics.upper(), n, nt),
+            "store": "wave-predicted (one propagator step + the source's image as predictor, quantised "
+                     "residuals, 16-bit keyframes, segments of %d frames)" % k,
+            "stored_ratio_vs_float32": round(4.0 * nf * n ** 3 * nt / stored, 3),
+            "image_rel_l2_vs_exact": rel, "seconds": {"exact_checkpointed": round(secs["exact"], 2),
+                                                      "wave_store": round(secs["wave"], 2)},
+            "target": ">= 8x fewer snapshot bytes with the Born gradient within rel L2 1e-3 of exact storage"}
 
 
 def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
@@ -679,18 +744,33 @@ def cpu_baseline(physics, n, target_s=12.0):
                 physics.upper(), n, k, dt)}
 
 
+# the sources the step kernels are built from: a traffic record applies to the build it measured
+KERNEL_SOURCES = ("wfd_fused.hpp", "wfd_wide.hpp", "wfd_kernels.hpp", "wfd_runtime.hpp")
+
+
+def kernel_source_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "wavefd_amd", "csrc", name), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(physics, n):
-    """HBM bytes per launch of the fused step kernel from the committed rocprofv3 --pmc runs
-    (profiles/k_fused_step_traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each
+    """Bytes per launch of the fused step kernel between the L2 and the memory fabric (so
+    Infinity-Cache hits included: an upper bound on HBM bytes) from the committed rocprofv3 --pmc
+    runs (profiles/k_fused_step_traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each
     scaled by what the counter reads for a plain copy kernel of known size), or None when no
-    measurement of this workload is committed."""
+    measurement of this workload on this build of the kernels is committed."""
     path = os.path.join(ROOT, "profiles", "k_fused_step_traffic.json")
     try:
         with open(path) as fh:
-            rec = json.load(fh).get("%s_%d" % (physics, n))
+            doc = json.load(fh)
     except (OSError, ValueError):
         return None
-    if not rec:
+    rec = doc.get("%s_%d" % (physics, n))
+    if not rec or doc.get("kernel_source_hash") != kernel_source_hash():
         return None
     return rec["fetch_kib"] * rec["fetch_scale"] * 1024 + rec["write_kib"] * rec["write_scale"] * 1024
 
@@ -717,7 +797,10 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=320)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extra", action="store_true", help="also time Iso 256^3 and VTI 512^3")
+    ap.add_argument("--no-extra", action="store_true", help="skip the Iso 256^3 and VTI 512^3 steps")
+    ap.add_argument("--no-born", action="store_true",
+                    help="skip the Born shots/hour (fused and separate calls, Iso and VTI, 24 shots each) and "
+                         "the compressed-snapshot gradient measurements of the default line")
     ap.add_argument("--born", choices=["iso", "vti"], default=None,
                     help="only measure Born shots/hour on ginsu windows of a 1000x1000x600 model")
     ap.add_argument("--born-shots", type=int, default=24)
@@ -765,12 +848,42 @@ def main():
               "build has no multi-GPU path; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    def progress(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
+
     res = run_gpu(args.physics, args.n, args.steps, args.warmup, tune=args.autotune, with_taps=True)
+    progress("%s %d^3 step: %.3f ms" % (args.physics, args.n, res["kernel_ms"]))
     extra = []
-    if args.extra:
+    if not args.no_extra:
         for ph, n in (("iso", 256), ("vti", 512)):
             r = run_gpu(ph, n, max(10, args.steps), args.warmup)
-            extra.append({k: r[k] for k in ("physics", "n", "mcells_s", "kernel_ms", "achieved_gbs")})
+            progress("%s %d^3 step: %.4f ms" % (ph, n, r["kernel_ms"]))
+            extra.append(dict({k: r[k] for k in ("physics", "n", "mcells_s", "kernel_ms", "achieved_gbs")},
+                              frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), bytes_per_cell=BYTES_PER_CELL[ph],
+                              launch=r["launch"]))
+    born_line = None
+    if not args.no_born:
+        born_line = {}
+        for ph, nt in (("iso", 200), ("vti", 100)):
+            fused = run_born(ph, 24, nt, fused=True)
+            progress("born %s fused: %.1f shots/h" % (ph, fused["shots_per_hour"]))
+            plain = run_born(ph, 24, nt, fused=False)
+            progress("born %s separate calls: %.1f shots/h" % (ph, plain["shots_per_hour"]))
+            born_line[ph] = {"shots": 24, "nt": nt, "shots_per_hour_fused": fused["shots_per_hour"],
+                             "shots_per_hour_separate_calls": plain["shots_per_hour"],
+                             "ratio": round(fused["shots_per_hour"] / plain["shots_per_hour"], 3),
+                             "wall_s_fused": fused["wall_s"], "wall_s_separate_calls": plain["wall_s"],
+                             "image_rel_l2_fused_vs_separate": 0.0 if fused["image_sha"] == plain["image_sha"]
+                             else None}
+        born_line["workload"] = ("24 shots on seeded ginsu windows (per-axis sizes U{200..600}) of a 1000 x 1000 x "
+                                 "600 model at 12.5 m: forward (DP snapshots, exact) + Born forward + adjoint "
+                                 "imaging into the shared image; fused: the per-step pairs as fused steps (Iso: "
+                                 "background and Born field in one pass); separate calls: the reference's API "
+                                 "calls one by one")
+        store = run_store()
+        progress("snapshot store: %.2fx, image rel L2 %.2e" % (store["stored_ratio_vs_float32"],
+                                                               store["image_rel_l2_vs_exact"]))
+        born_line["snapshot_compression"] = store
     if args.coherent:
         for ph, n in (("iso", 256), (args.physics, min(args.n, 512))):
             extra.append({"coherent": run_coherent(ph, n, 5)})
@@ -819,6 +932,8 @@ def main():
         line["autotune"] = res["autotune"]
     if extra:
         line["extra"] = extra
+    if born_line:
+        line["born"] = born_line
     print(json.dumps(line), flush=True)
 
 
