@@ -21,6 +21,7 @@ import argparse
 import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -704,33 +705,64 @@ def run_codec(n=128, nt=300, iters=800, seed=7, big_n=512):
             "target": ">= 8x fewer snapshot bytes with the Born gradient within rel L2 1e-3 of exact storage"}
 
 
-def cpu_baseline(physics, n, target_s=12.0):
-    """Time the oracle's two-pass step (the reference's structure, nbx=nby=8, nbz=nz) on the
-    host cores for about target_s seconds on an n^3 grid; returns Mcells/s."""
+def _host_info():
+    """nproc, the CPUs this process may run on, and lscpu's model/socket/core lines"""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+                               "NUMA node(s)"):
+                info["lscpu " + key.strip()] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        info["lscpu"] = "unavailable"
+    return info
+
+
+def _cpu_child(physics, n, target_s):
+    """(child process) the oracle's two-pass step on the configs' n^3 model, FTZ/DAZ on"""
     from oracle import oracle as orc
+    from tests.helpers import config_model, smooth_noise_big
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     cls = {"iso": orc.OracleIso, "vti": orc.OracleVTI, "tti": orc.OracleTTI}[physics]
-    prop = cls(n, n, n, 12.5, 12.5, 12.5, dt=1e-3, fs=True, nsponge=20, freqQ=10.0, qMin=0.1, qInterior=100.0,
+    d, dt = 12.5, 1e-3
+    nsponge = 50 if physics == "tti" else 40
+    t0 = time.perf_counter()
+    prop = cls(n, n, n, d, d, d, dt=dt, fs=True, nsponge=nsponge, freqQ=10.0, qMin=0.1, qInterior=100.0,
                nthread=threads)
-    # timing does not depend on the values (no denormals, no NaN): a uniform random model
-    rng = np.random.default_rng(5)
-    prop.v[...] = rng.uniform(1500, 3000, prop.v.shape).astype(np.float32)
-    prop.b[...] = rng.uniform(0.4, 1.0, prop.b.shape).astype(np.float32)
-    if physics != "iso":
-        prop.eps[...] = rng.uniform(0.0, 0.3, prop.v.shape).astype(np.float32)
-        prop.eta[...] = rng.uniform(0.0, 0.15, prop.v.shape).astype(np.float32)
-        prop.f[...] = 0.85
+    m = config_model(physics, (n, n, n), d, dt, 3)
+    for name, arr in m.items():
+        getattr(prop, name)[...] = arr
+    del m
     if physics == "tti":
-        th = rng.uniform(-0.78, 0.78, prop.v.shape)
-        ph = rng.uniform(0, 6.28, prop.v.shape)
+        # tilt in [-45, 45] deg and azimuth in [0, 360) deg, smoothed by 25 cells (azimuth as a unit
+        # vector); Q from 30 to 200 in the interior, the sponge keeping SetupDtOmegaInvQ's profile
+        th = (np.pi / 4) * smooth_noise_big((n, n, n), 25, 303)
         prop.sinTheta[...] = np.sin(th)
         prop.cosTheta[...] = np.cos(th)
-        prop.sinPhi[...] = np.sin(ph)
-        prop.cosPhi[...] = np.cos(ph)
+        del th
+        c = smooth_noise_big((n, n, n), 25, 304)
+        s = smooth_noise_big((n, n, n), 25, 305)
+        r = np.sqrt(c * c + s * s)
+        np.maximum(r, 1e-6, out=r)
+        prop.cosPhi[...] = c / r
+        prop.sinPhi[...] = s / r
+        del c, s, r
+        k = np.arange(n)
+        dist = np.minimum(k, n - 1 - k)
+        ks = np.minimum(np.minimum(dist[:, None, None], dist[None, :, None]), (n - 1 - k)[None, None, :])
+        q = (dt * 2 * np.pi * 10.0 / (115.0 + 85.0 * smooth_noise_big((n, n, n), 15, 306))).astype(np.float32)
+        np.copyto(prop.q, q, where=ks >= nsponge)
+        del q, ks
+    p0 = smooth_noise_big((n, n, n), 4, 307)
     for name in ("pCur", "pOld") + (("mCur", "mOld") if physics != "iso" else ()):
-        getattr(prop, name)[...] = rng.uniform(-1, 1, prop.v.shape).astype(np.float32)
-    prop.time_step()  # warm-up (first touch)
+        getattr(prop, name)[...] = p0
+    del p0
+    setup_s = time.perf_counter() - t0
+    orc.lib().wo_set_ftz(threads, 1)
+    prop.time_step()  # warm-up (first touch of the temporaries)
     t0 = time.perf_counter()
     k = 0
     while True:
@@ -738,10 +770,27 @@ def cpu_baseline(physics, n, target_s=12.0):
         k += 1
         if time.perf_counter() - t0 >= target_s:
             break
-    dt = time.perf_counter() - t0
-    return {"value": n ** 3 * k / dt / 1e6, "unit": "Mcells/s", "cores": threads, "kind": "port",
-            "sample": "oracle two-pass %s step (reference structure) on %d^3, %d steps in %.1f s" % (
-                physics.upper(), n, k, dt)}
+    el = time.perf_counter() - t0
+    return {"value": n ** 3 * k / el / 1e6, "unit": "Mcells/s", "cores": threads, "kind": "port",
+            "sample": "oracle two-pass %s step (the reference's structure, nbx = nby = 8, nbz = nz) on the "
+                      "configs' %d^3 model, %d steps in %.1f s after one warm-up step" % (physics.upper(), n, k, el),
+            "settings": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+                         "OMP_PLACES": os.environ.get("OMP_PLACES"), "ftz_daz": True,
+                         "model_setup_s": round(setup_s, 1)},
+            "host": _host_info()}
+
+
+def cpu_baseline(physics, n, target_s=12.0):
+    """Time the oracle's two-pass step (the reference's structure) on the host cores for about
+    target_s seconds on the configs' n^3 model, in a child process started with OMP_PROC_BIND=close
+    and OMP_PLACES=cores (benchmark/README.md:8) and flush-to-zero on; returns its JSON record."""
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", "--physics", physics, "--cpu-n", str(n),
+           "--cpu-seconds", str(target_s)]
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+    if out.returncode != 0:
+        raise RuntimeError("CPU baseline failed:\n" + out.stderr[-2000:])
+    return json.loads(out.stdout.strip().splitlines()[-1])
 
 
 # the sources the step kernels are built from: a traffic record applies to the build it measured
@@ -794,7 +843,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--physics", default="tti", choices=["iso", "vti", "tti"])
     ap.add_argument("--n", type=int, default=800)
-    ap.add_argument("--cpu-n", type=int, default=320)
+    ap.add_argument("--cpu-n", type=int, default=800)
+    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the Iso 256^3 and VTI 512^3 steps")
@@ -826,6 +876,9 @@ def main():
                     help="only run a plain 1 GiB device-to-device copy kernel 3 times (the known-size "
                          "kernel that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.cpu_child:
+        print(json.dumps(_cpu_child(args.physics, args.cpu_n, args.cpu_seconds)), flush=True)
+        return
     if args.copy_probe:
         copy_probe()
         return

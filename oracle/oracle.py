@@ -86,6 +86,7 @@ def _declare(l):
     l.wo_interp_forward.argtypes = [_fp, _fp, _fp, _L, _L, _L, _L]
     l.wo_interp_adjoint.argtypes = [_fp, _fp, _fp, _L, _L, _L, _L]
     l.wo_max_threads.restype = ctypes.c_int
+    l.wo_set_ftz.argtypes = [_L, ctypes.c_int]
 
 
 def _p(a):

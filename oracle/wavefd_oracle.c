@@ -39,6 +39,7 @@
 #include <float.h>
 #include <stddef.h>
 #include <omp.h>
+#include <xmmintrin.h>
 
 /* Stencil coefficients: double expressions stored to float, as the reference's
  * constructor initialiser list does (src/prop3DAcoIsoDenQ_DEO2_FDTD.h:64-67). */
@@ -1037,4 +1038,19 @@ void wo2_tti_update(long fs, long nx, long nz, float idx, float idz, float dt, c
             pOld[k] = dt2V2_B * pSpace[k] - q[k] * (pCur[k] - pOld[k]) - pOld[k] + 2 * pCur[k];
             mOld[k] = dt2V2_B * mSpace[k] - q[k] * (mCur[k] - mOld[k]) - mOld[k] + 2 * mCur[k];
         }
+}
+
+/* Flush-to-zero and denormals-are-zero on every thread of an nthread OpenMP team (MXCSR bits 15
+ * and 6): the reference's timing loop runs with set_zero_subnormals(true)
+ * (src/prop3DAcoTTIDenQ_DEO2_FDTD.jl:172).  For timing only; the parity tests keep IEEE
+ * subnormals (results differ only where a value would be subnormal). */
+void wo_set_ftz(long nthread, int on) {
+#pragma omp parallel num_threads(nthread)
+    {
+        unsigned m = _mm_getcsr();
+        m = on ? (m | 0x8040u) : (m & ~0x8040u);
+        _mm_setcsr(m);
+    }
+    unsigned m = _mm_getcsr();
+    _mm_setcsr(on ? (m | 0x8040u) : (m & ~0x8040u));
 }
