@@ -722,6 +722,7 @@ def _host_info():
 
 def _cpu_child(physics, n, target_s):
     """(child process) the oracle's two-pass step on the configs' n^3 model, FTZ/DAZ on"""
+    host = _host_info()  # before OpenMP binds this thread to one place
     from oracle import oracle as orc
     from tests.helpers import config_model, smooth_noise_big
 
@@ -777,7 +778,7 @@ def _cpu_child(physics, n, target_s):
             "settings": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
                          "OMP_PLACES": os.environ.get("OMP_PLACES"), "ftz_daz": True,
                          "model_setup_s": round(setup_s, 1)},
-            "host": _host_info()}
+            "host": host}
 
 
 def cpu_baseline(physics, n, target_s=12.0):

@@ -44,9 +44,11 @@ def test_results_do_not_depend_on_the_configuration(kind):
     shape = (80, 40, 70)
     _, ref = _run(kind, shape, None)
     for config in (("64x16", False, 0), ("32x32", False, 0), ("64x16", False, 27), ("32x32", False, 14),
-                   ("64x16", False, 80), ("wide", False, 0), ("wide", False, 9)):
+                   ("64x16", False, 80), ("wide", False, 0), ("wide", False, 9), ("wide2", False, 0),
+                   ("wide2", False, 11)):
         eff, got = _run(kind, shape, config)
-        assert eff["tile"] == config[0]
+        # "wide2" is the plain Iso step's; the other physics take "wide" for it
+        assert eff["tile"] == (config[0] if kind == "iso" or config[0] != "wide2" else "wide")
         if config[2]:
             assert eff["chunk"] == config[2]
         for a, b in zip(ref, got):
@@ -71,7 +73,7 @@ def test_rounds_in_step_change_no_result(kind, tile, chunk):
 def test_set_fused_config_rejects_bad_values():
     import wavefd_amd as w
     p = _prop("iso", (24, 20, 22))
-    for bad in ((4, 0, 0), (-1, 0, 0), (0, 2, 0), (0, 0, -1)):
+    for bad in ((5, 0, 0), (-1, 0, 0), (0, 2, 0), (0, 0, -1)):
         assert p._lib.SetFusedConfig(p.p, *bad) == -1
         p._lib.ClearError()
     w.free(p)

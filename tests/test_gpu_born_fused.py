@@ -5,8 +5,9 @@ TimeStepBornInjection_V = TimeStep; ForwardBornInjection_V       (new pCur, mCur
 TimeStepAdjointBornAccumulation_V = TimeStep; AdjointBornAccumulation_V (g, new pCur, mCur)
 
 for Iso, VTI and TTI, with and without a free surface, on a grid whose tiles are ragged, with
-the fused kernel's two tile shapes and with the two-pass fallback (SetFusedStep 0); and the Born driver gives the
-same Born data and image with the fused pairs as with the separate calls.
+the fused kernel's tile shapes (64x16, 32x32, wide, wide2) and with the two-pass fallback
+(SetFusedStep 0); and the Born driver gives the same Born data and image with the fused pairs
+as with the separate calls.
 """
 import numpy as np
 import pytest
@@ -43,7 +44,7 @@ def _arrs(kind, seed):
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [True, False])
-@pytest.mark.parametrize("fused", [1, 2, 0])  # tile 64x16, tile 32x32, two-pass
+@pytest.mark.parametrize("fused", [1, 2, 3, 4, 0])  # tile 64x16, 32x32, wide, wide2, two-pass
 def test_save_dp(kind, fs, fused):
     a, b = _prop(kind, fs, fused), _prop(kind, fs, fused)
     a._lib.TimeStep(a.p)
@@ -59,7 +60,7 @@ def test_save_dp(kind, fs, fused):
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [True, False])
-@pytest.mark.parametrize("fused", [1, 2, 0])  # tile 64x16, tile 32x32, two-pass
+@pytest.mark.parametrize("fused", [1, 2, 3, 4, 0])  # tile 64x16, 32x32, wide, wide2, two-pass
 def test_born_injection(kind, fs, fused):
     a, b = _prop(kind, fs, fused), _prop(kind, fs, fused)
     rng = np.random.default_rng(8)
@@ -74,7 +75,7 @@ def test_born_injection(kind, fs, fused):
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [True, False])
-@pytest.mark.parametrize("fused", [1, 2, 0])  # tile 64x16, tile 32x32, two-pass
+@pytest.mark.parametrize("fused", [1, 2, 3, 4, 0])  # tile 64x16, 32x32, wide, wide2, two-pass
 def test_adjoint_accumulation(kind, fs, fused):
     a, b = _prop(kind, fs, fused), _prop(kind, fs, fused)
     rng = np.random.default_rng(10)
@@ -91,7 +92,7 @@ def test_adjoint_accumulation(kind, fs, fused):
 
 
 @pytest.mark.parametrize("fs", [True, False])
-@pytest.mark.parametrize("fused", [1, 2, 0])  # tile 64x16, tile 32x32, two-pass
+@pytest.mark.parametrize("fused", [1, 2, 3, 4, 0])  # tile 64x16, 32x32, wide, wide2, two-pass
 def test_coupled_step(fs, fused):
     """TimeStepBornCoupled_V on one handle = TimeStepSaveDP on a background handle followed by
     TimeStepBornInjection_V on a handle holding the Born field, step after step"""

@@ -55,7 +55,7 @@ def test_q_profile_matches_oracle(oracle, kind, fs):
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [False, True])
-@pytest.mark.parametrize("fused", ["64x16", "32x32", "wide", None])
+@pytest.mark.parametrize("fused", ["64x16", "32x32", "wide", "wide2", None])
 @pytest.mark.parametrize("shape", SHAPES)
 def test_steps_bit_exact(oracle, kind, fs, fused, shape):
     import wavefd_amd as w
@@ -72,12 +72,13 @@ def test_steps_bit_exact(oracle, kind, fs, fused, shape):
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("chunk", [5, 13])
-def test_wide_tile_short_chunks_bit_exact(oracle, kind, chunk):
+@pytest.mark.parametrize("tile", ["wide", "wide2"])
+def test_wide_tile_short_chunks_bit_exact(oracle, kind, chunk, tile):
     # several x-chunks per grid, each padded to whole 8-step rounds of the wide kernel's loop
     import wavefd_amd as w
     shape = (37, 41, 70)
     prop, orc = _pair(oracle, kind, shape, True, 5)
-    w.set_fused_config(prop, "wide", False, chunk)
+    w.set_fused_config(prop, tile, False, chunk)
     for _ in range(3):
         w.propagateforward(prop)
         orc.time_step()

@@ -567,15 +567,15 @@ __global__ void __launch_bounds__(ZW *NROW) k_fused_batch(FusedBatch<PHYS> B) {
         fused_body<PHYS, ZW, NROW, EPI, false>(A, S, zt, yt, xc);
 }
 
-// x-planes per chunk: balance whole rounds of workgroups (one per CU) against the 7-step
+// x-planes per chunk: balance whole rounds of workgroups (per_cu per CU) against the 7-step
 // warm-up and the padding to a multiple of `unroll` steps that each chunk pays
-static inline int fused_cx(const Prop *p, long tiles, int unroll = 4) {
+static inline int fused_cx(const Prop *p, long tiles, int unroll = 4, int per_cu = 1) {
     if (p->fused_chunk > 0) return (int)std::min<long>(p->fused_chunk, p->nx);
     if (const char *e = std::getenv("WAVEFD_FUSED_CX")) {
         const long v = std::atol(e);
         if (v > 0) return (int)std::min<long>(v, p->nx);
     }
-    const long cus = p->cu_count > 0 ? p->cu_count : 256;
+    const long cus = (p->cu_count > 0 ? p->cu_count : 256) * per_cu;
     long best_cx = p->nx;
     double best = 1e300;
     for (long nxc = 1; nxc <= 64 && nxc <= p->nx; nxc++) {
@@ -704,34 +704,39 @@ static inline bool fused_fits(Prop *p) {
 
 // tile shapes of the fused kernel (temporaries along z x along y; outputs (z-7) x (y-7)):
 // 64 x 16 and 32 x 32 with one thread per cell (k_fused_step), 64 x 32 with four y-rows per
-// thread for Iso, 64 x 24 with three for VTI and TTI (k_wide_step)
-enum FusedTile { TILE_AUTO = 0, TILE_64x16 = 1, TILE_32x32 = 2, TILE_64x32W = 3 };
+// thread for Iso, 64 x 24 with three for VTI and TTI (k_wide_step); 64 x 16 with two y-rows per
+// thread and two workgroups per CU for the one-wavefield steps (Iso's; the others take tile 3 for
+// it)
+enum FusedTile { TILE_AUTO = 0, TILE_64x16 = 1, TILE_32x32 = 2, TILE_64x32W = 3, TILE_64x16W = 4 };
 
 static inline bool set_fused_tile(Prop *p, long tile) {
-    if (tile < TILE_AUTO || tile > TILE_64x32W) {
-        set_error("SetFusedTile: tile must be 0 (automatic), 1 (64x16), 2 (32x32) or 3 (wide: 64x32 Iso, 64x24 VTI/TTI)");
+    if (tile < TILE_AUTO || tile > TILE_64x16W) {
+        set_error("SetFusedTile: tile must be 0 (automatic), 1 (64x16), 2 (32x32), 3 (wide: 64x32 Iso, 64x24 "
+                  "VTI/TTI) or 4 (wide 64x16, two workgroups per CU: Iso's one-wavefield steps)");
         return false;
     }
     p->fused_tile = (int)tile;
     return true;
 }
 
-// automatic choice (measured on MI355X, 800^3 TTI, 512^3 VTI, 256^3 Iso).  Plain VTI and TTI
+// automatic choice (measured on MI355X, 800^3 TTI, 512^3 VTI, 256^3 and 512^3 Iso).  VTI and TTI
 // steps: the wide tile (3 y-rows per thread, 57 x 17 outputs), whose straight-line rows give each
 // wave the instruction-level parallelism the 64 x 16 tile gets from more waves (VTI 512^3 2.34
-// against 2.72 ms, TTI 800^3 12.39 against 12.69 ms).  Plain Iso step: the square 32 x 32 tile,
-// which recomputes fewer halo temporaries per output than 64 x 16 (1024 / 625 threads per output
-// against 1024 / 504; 0.168 against 0.223 ms at 256^3; the wide tile 0.180 ms)
+// against 2.72 ms, TTI 800^3 12.39 against 12.69 ms; the Born epilogues, 24 VTI shots on
+// 200^3..600^3 windows: 7938 shots per hour against 7300 (64 x 16) and 6768 (32 x 32)).  Iso's
+// one-wavefield steps: the two-row wide tile, two workgroups per CU (512^3 1.097 against 1.328 ms
+// for the 32 x 32 tile and 1.50 for 64 x 32 wide, 256^3 0.174 against 0.179 ms; the 24 Iso Born
+// shots 8299 against 8000 shots per hour with the wide tile's epilogues, the separate calls 6184
+// against 5583).  The coupled Iso Born step (two wavefields, 167 VGPRs at two rows: one workgroup
+// per CU) keeps the wide tile.
 static inline int fused_tile_of(const Prop *p, int epi) {
+    if (p->fused_tile == TILE_64x16W && fused_nf(p->phys, epi) != 1) return TILE_64x32W;
     if (p->fused_tile != TILE_AUTO) return p->fused_tile;
-    if (epi == EPI_NONE) return p->phys == ISO ? TILE_32x32 : TILE_64x32W;
-    // the Born epilogues (1 to 3 more fields per output) on the wide tile too: 24 Born shots on
-    // 200^3..600^3 windows, Iso 8015 against 7746 shots per hour (64 x 16), VTI 7912 against 7240
-    return TILE_64x32W;
+    return fused_nf(p->phys, epi) == 1 ? TILE_64x16W : TILE_64x32W;
 }
 
 // the fused step's launch configuration (tile, rounds in step, x-chunk length); results do not
-// depend on it.  tile 0..2 as SetFusedTile, lockstep -1 automatic / 0 / 1, chunk 0 automatic.
+// depend on it.  tile 0..4 as SetFusedTile, lockstep -1 automatic / 0 / 1, chunk 0 automatic.
 static inline bool set_fused_config(Prop *p, long tile, long lockstep, long chunk) {
     if (lockstep < -1 || lockstep > 1 || chunk < 0) {
         set_error("SetFusedConfig: lockstep must be -1, 0 or 1 and chunk >= 0");
@@ -743,7 +748,7 @@ static inline bool set_fused_config(Prop *p, long tile, long lockstep, long chun
     return true;
 }
 
-// the configuration a plain TimeStep launches with: out[0] tile (1 or 2), out[1] lockstep
+// the configuration a plain TimeStep launches with: out[0] tile (1 to 4), out[1] lockstep
 // (0 or 1), out[2] x-planes per chunk, out[3] workgroups
 static inline bool get_fused_config(const Prop *p, long *out) {
     if (!out) {
@@ -754,7 +759,9 @@ static inline bool get_fused_config(const Prop *p, long *out) {
     const int zo = (tile == TILE_32x32 ? 32 : 64) - 7;
     const int yo = (tile == TILE_32x32 ? 32 : (tile == TILE_64x32W ? wide_rows(p->phys, EPI_NONE) : 16)) - 7;
     const long tiles = ((p->nz + zo - 1) / zo) * ((p->ny + yo - 1) / yo);
-    const long cx = tile == TILE_64x32W ? wide_cx(p, tiles, p->phys, EPI_NONE) : fused_cx(p, tiles, 4);
+    const long cx = tile == TILE_64x32W   ? wide_cx(p, tiles, p->phys, EPI_NONE)
+                    : tile == TILE_64x16W ? wide_cx2(p, tiles, p->phys, EPI_NONE)
+                                          : fused_cx(p, tiles, 4);
     const long ntiles = tiles * ((p->nx + cx - 1) / cx);
     out[0] = tile;
     out[1] = fused_lockstep_of(p, p->phys, EPI_NONE, ntiles) ? 1 : 0;
@@ -766,6 +773,8 @@ static inline bool get_fused_config(const Prop *p, long *out) {
 template <int PHYS, int EPI>
 static inline bool launch_fused_tile(Prop *p, bool linear, float *e0, float *e1, float *e2) {
     const int tile = fused_tile_of(p, EPI);
+    if constexpr (fused_nf(PHYS, EPI) == 1)
+        if (tile == TILE_64x16W) return launch_wide<PHYS, EPI, 2>(p, linear, e0, e1, e2);
     if (tile == TILE_64x32W) return launch_wide<PHYS, EPI>(p, linear, e0, e1, e2);
     if (tile == TILE_32x32) return launch_fused<PHYS, 32, 32, EPI>(p, linear, e0, e1, e2);
     return launch_fused<PHYS, 64, 16, EPI>(p, linear, e0, e1, e2);
@@ -821,8 +830,11 @@ static inline bool launch_fused_batch(Prop **ps, int n, float *const *e0, float 
 
 template <int PHYS, int EPI>
 static inline bool launch_fused_batch_tile(Prop **ps, int n, float *const *e0, float *const *e1, float *const *e2) {
-    // (the several-grid launch has the one-thread-per-cell tiles only; the wide tile runs as 64 x 16)
-    if (fused_tile_of(ps[0], EPI) == TILE_32x32) return launch_fused_batch<PHYS, 32, 32, EPI>(ps, n, e0, e1, e2);
+    // the several-grid launch has the one-thread-per-cell tiles only: the wide tile runs as 64 x 16,
+    // the two-row wide tile of the plain Iso step as 32 x 32 (that step's tile before it)
+    const int tile = fused_tile_of(ps[0], EPI);
+    if (tile == TILE_32x32 || (EPI == EPI_NONE && tile == TILE_64x16W))
+        return launch_fused_batch<PHYS, 32, 32, EPI>(ps, n, e0, e1, e2);
     return launch_fused_batch<PHYS, 64, 16, EPI>(ps, n, e0, e1, e2);
 }
 

@@ -58,6 +58,10 @@ constexpr int WIDE_THREADS = 512;  // 64 lanes x 8 waves
 // 160 KiB of LDS, at 4 rows
 constexpr int wide_r(int phys, int epi) { return fused_nf(phys, epi) == 1 ? 4 : 3; }
 constexpr int wide_rows(int phys, int epi) { return 8 * wide_r(phys, epi); }
+// waves per SIMD the kernel is built for: 2 (one 512-thread workgroup per CU, up to 256 VGPRs), or
+// 4 for the one-wavefield tile of 2 rows per thread (two workgroups per CU, 128 VGPRs; 105 used,
+// 46,856 B of LDS each)
+constexpr int wide_wpe(int nf, int r) { return nf == 1 && r == 2 ? 4 : 2; }
 
 template <int PHYS, int R, int EPI, bool FST>
 __device__ __forceinline__ void wide_body(const FusedArgs<PHYS> &A, WideLds<fused_nf(PHYS, EPI), R> &S, int zt,
@@ -497,9 +501,8 @@ __device__ __forceinline__ void wide_body(const FusedArgs<PHYS> &A, WideLds<fuse
     }
 }
 
-template <int PHYS, int EPI>
-__global__ void __launch_bounds__(WIDE_THREADS) k_wide_step(FusedArgs<PHYS> A) {
-    constexpr int R = wide_r(PHYS, EPI);
+template <int PHYS, int EPI, int R>
+__global__ void __launch_bounds__(WIDE_THREADS, wide_wpe(fused_nf(PHYS, EPI), R)) k_wide_step(FusedArgs<PHYS> A) {
     __shared__ WideLds<fused_nf(PHYS, EPI), R> S;
     const int bid = (int)blockIdx.x, xcd = bid & 7;
     int loc, zt, yt, xc;
@@ -545,17 +548,22 @@ static inline int wide_cx(const Prop *p, long tiles, int phys, int epi) {
     return fused_cx(p, tiles, 8);
 }
 
-template <int PHYS, int EPI>
+// the same rule for two resident workgroups per CU
+static inline int wide_cx2(const Prop *p, long tiles, int, int) { return fused_cx(p, tiles, 8, 2); }
+
+template <int PHYS, int EPI, int R = wide_r(PHYS, EPI)>
 static inline bool launch_wide(Prop *p, bool linear, float *e0, float *e1, float *e2) {
     FusedArgs<PHYS> A;
-    const long ntiles = fill_fused<PHYS, 64, wide_rows(PHYS, EPI), EPI>(p, linear, e0, e1, e2, A, 8, wide_cx);
+    constexpr bool two_per_cu = wide_wpe(fused_nf(PHYS, EPI), R) == 4;
+    const long ntiles =
+        fill_fused<PHYS, 64, 8 * R, EPI>(p, linear, e0, e1, e2, A, 8, two_per_cu ? wide_cx2 : wide_cx);
     const bool lockstep = fused_lockstep_of(p, PHYS, EPI, ntiles);
     if (lockstep && p->fused_sync) {
         A.per_xcd = p->cu_count / 8;
         k_zero_u32<<<1, 64, 0, p->stream>>>(p->fused_sync, 8);
         WFD_LAUNCHED();
     }
-    k_wide_step<PHYS, EPI><<<dim3((unsigned)ntiles), dim3(WIDE_THREADS), 0, p->stream>>>(A);
+    k_wide_step<PHYS, EPI, R><<<dim3((unsigned)ntiles), dim3(WIDE_THREADS), 0, p->stream>>>(A);
     return true;
 }
 

@@ -222,12 +222,15 @@ def set_fused_step(prop, fused=True):
     prop._lib.SetFusedStep(prop.p, 1 if fused else 0)
 
 
-FUSED_TILES = {"auto": 0, "64x16": 1, "32x32": 2, "wide": 3}
+FUSED_TILES = {"auto": 0, "64x16": 1, "32x32": 2, "wide": 3, "wide2": 4}
 
 
 def set_fused_tile(prop, tile="auto"):
     """Tile shape of the fused step kernel: "64x16" (64 z-lanes x 16 y-rows per workgroup,
-    56 x 9 outputs), "32x32" (25 x 25 outputs) or "auto".  Results do not depend on it."""
+    57 x 9 outputs), "32x32" (25 x 25 outputs), "wide" (512 threads with 4 y-rows each for Iso,
+    57 x 25 outputs; 3 for VTI and TTI, 57 x 17), "wide2" (Iso's one-wavefield steps: 2 y-rows
+    per thread, 57 x 9 outputs, two workgroups per CU; other steps take "wide") or "auto".
+    Results do not depend on it."""
     if prop._lib.SetFusedTile(prop.p, FUSED_TILES[tile]) != 0:
         prop._check()
 

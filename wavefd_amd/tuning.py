@@ -70,12 +70,12 @@ def fused_config(prop):
     out = (ctypes.c_long * 4)()
     if prop._lib.GetFusedConfig(prop.p, out) != 0:
         prop._check()
-    tile = {1: "64x16", 2: "32x32", 3: "wide"}[out[0]]
+    tile = {1: "64x16", 2: "32x32", 3: "wide", 4: "wide2"}[out[0]]
     return {"tile": tile, "lockstep": bool(out[1]), "chunk": int(out[2]), "workgroups": int(out[3])}
 
 
 def set_fused_config(prop, tile="auto", lockstep=None, chunk=0):
-    """Set the fused step's launch configuration: tile "auto" | "64x16" | "32x32", lockstep
+    """Set the fused step's launch configuration: tile "auto" | "64x16" | "32x32" | "wide" | "wide2", lockstep
     None (automatic) | False | True, chunk 0 (automatic) or x-planes per chunk."""
     ls = -1 if lockstep is None else (1 if lockstep else 0)
     if prop._lib.SetFusedConfig(prop.p, _p3.FUSED_TILES[tile], ls, int(chunk)) != 0:
@@ -83,11 +83,11 @@ def set_fused_config(prop, tile="auto", lockstep=None, chunk=0):
 
 
 def _candidates(prop):
-    """Distinct effective configurations: both tiles, rounds in step on and off, and the
+    """Distinct effective configurations: every tile, rounds in step on and off, and the
     x-chunk lengths that split nx into 1, 2, 3, 4, 6 or 8 chunks besides the automatic one."""
     nx = prop.shape[2]
     seen, out = set(), []
-    for tile in ("64x16", "32x32", "wide"):
+    for tile in ("64x16", "32x32", "wide", "wide2"):
         for lockstep in (False, True):
             for nxc in (0, 1, 2, 3, 4, 6, 8):
                 chunk = 0 if nxc == 0 else -(-nx // nxc)
