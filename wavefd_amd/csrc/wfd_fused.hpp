@@ -166,7 +166,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<fu
     const int xe = xs + A.cx < nx ? xs + A.cx : nx;
     const int zlo = fst ? 0 : 4;
     const bool yzU = y >= 4 && y < ny - 4 && z >= zlo && z < nz - 4;
-    const bool out_thread = l >= 4 && l < 4 + ZO && r >= 4 && r < 4 + YO && y < ny && z < nz;
+    // outputs: the inner (ZW - 7) x (NROW - 7) cells; the last tile along an axis also writes the
+    // grid's last cells that lie in its upper halo -- annulus cells (fill_fused counts the tiles
+    // so), whose new values need no D- (L = 0 there), only their own operands
+    const bool out_thread = l >= 4 && (l < 4 + ZO || zt == A.nzt - 1) && r >= 4 &&
+                            (r < 4 + YO || yt == A.nyt - 1) && y < ny && z < nz;
     uint32_t own_sign;
     bool own_ok;
     const uint32_t own = col_off(y, z, ny, nz, fst, own_sign, own_ok);
@@ -591,6 +595,12 @@ static inline int fused_cx(const Prop *p, long tiles, int unroll = 4, int per_cu
     return (int)best_cx;
 }
 
+// tiles along z (y) for o outputs per tile: enough that the outputs cover the update region
+// [.., n - 4) and the last tile's upper halo (3 cells) reaches the grid's end, i.e. o * tiles >=
+// n - 3: the cells past the outputs are then annulus cells, which the last tile writes from its
+// halo lanes (rows).  Against ceil(n / o): 800 cells take 14 tiles of 57 (not 15), 47 of 17 (not 48)
+static inline long fused_tiles_along(long n, int o) { return std::max(1L, (n - 3 + o - 1) / o); }
+
 // e0, e1, e2: the epilogue's arrays (device), see Epilogue: SAVE_DP (DP, DM, -),
 // BORN_INJ (dV, DP, DM), IMAGE (g, DP, DM).  Returns the number of tiles.
 template <int PHYS, int ZW, int NROW, int EPI>
@@ -618,8 +628,8 @@ static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float 
         A.e_dv = e0;
         A.e_dp = e1;
     }
-    A.nzt = (int)((p->nz + ZO - 1) / ZO);
-    A.nyt = (int)((p->ny + YO - 1) / YO);
+    A.nzt = (int)fused_tiles_along(p->nz, ZO);
+    A.nyt = (int)fused_tiles_along(p->ny, YO);
     A.cx = cx_rule ? cx_rule(p, (long)A.nzt * A.nyt, PHYS, EPI) : fused_cx(p, (long)A.nzt * A.nyt, unroll);
     // blocks of (CUs per XCD) tiles, 4 along z
     const int xcd_cus = std::max(1, p->cu_count / 8);
@@ -758,7 +768,7 @@ static inline bool get_fused_config(const Prop *p, long *out) {
     const int tile = fused_tile_of(p, EPI_NONE);
     const int zo = (tile == TILE_32x32 ? 32 : 64) - 7;
     const int yo = (tile == TILE_32x32 ? 32 : (tile == TILE_64x32W ? wide_rows(p->phys, EPI_NONE) : 16)) - 7;
-    const long tiles = ((p->nz + zo - 1) / zo) * ((p->ny + yo - 1) / yo);
+    const long tiles = fused_tiles_along(p->nz, zo) * fused_tiles_along(p->ny, yo);
     const long cx = tile == TILE_64x32W   ? wide_cx(p, tiles, p->phys, EPI_NONE)
                     : tile == TILE_64x16W ? wide_cx2(p, tiles, p->phys, EPI_NONE)
                                           : fused_cx(p, tiles, 4);

@@ -88,8 +88,14 @@ __device__ __forceinline__ void wide_body(const FusedArgs<PHYS> &A, WideLds<fuse
     const int zlo = FST ? 0 : 4;
     const bool zU = z >= zlo && z < nz - 4;
     const bool zOut = l >= 4 && l < 4 + ZO && z < nz;
-    // per row: y inside the update region (uniform), output row (uniform), column offsets
-    bool yU[R], rOut[R], rOk[R];
+    // the last tile along an axis also writes the grid's last cells that lie in its upper halo
+    // (lanes 61..63, rows 4 + YO ..): these are annulus cells (fill_fused counts the tiles so),
+    // whose new values need no D- (L = 0 there), only their own operands
+    const bool zW = zOut || (zt == A.nzt - 1 && l >= 4 + ZO && z < nz);
+    const bool lastY = yt == A.nyt - 1;
+    // per row: y inside the update region (uniform), output row (uniform), written row (uniform),
+    // column offsets
+    bool yU[R], rOut[R], rW[R], rOk[R];
     uint32_t own[R], moff[R], ooff[R];
     uint32_t own_sign = 0;
 #pragma unroll
@@ -97,15 +103,19 @@ __device__ __forceinline__ void wide_body(const FusedArgs<PHYS> &A, WideLds<fuse
         const int y = yb + i;
         yU[i] = y >= 4 && y < ny - 4;
         rOut[i] = rb + i >= 4 && rb + i < 4 + YO && y < ny;
+        rW[i] = rOut[i] || (lastY && rb + i >= 4 + YO && y < ny);
         bool ok;
         own[i] = col_off(y, z, ny, nz, FST, own_sign, ok);
         rOk[i] = ok;
         moff[i] = (yU[i] && zU) ? own[i] : 0u;
-        ooff[i] = (rOut[i] && zOut) ? own[i] : 0u;
+        ooff[i] = (rW[i] && zW) ? own[i] : 0u;
     }
-    bool anyOut = false;  // the wave holds an output row (uniform)
+    bool anyOut = false, anyW = false;  // the wave holds an output (a written) row (uniform)
 #pragma unroll
-    for (int i = 0; i < R; i++) anyOut = anyOut || rOut[i];
+    for (int i = 0; i < R; i++) {
+        anyOut = anyOut || rOut[i];
+        anyW = anyW || rW[i];
+    }
     // border cells of the staged plane owned by this thread
     uint32_t boff[NBT], bsign[NBT];
     bool b_ok[NBT], has_b[NBT];
@@ -363,47 +373,54 @@ __device__ __forceinline__ void wide_body(const FusedArgs<PHYS> &A, WideLds<fuse
         WIDE_FENCE_OFF();
         // ---- 5. D- and leapfrog at plane x (temporaries of x written three steps ago) ----
         const bool xOut = x >= xs && x < xe;
-        if (xOut && anyOut && zOut) {
+        if (xOut && anyW && zW) {
             const bool xU = x >= 4 && x < nx - 4;
-            float ywin[NF][R + 7];  // ring rows rb-4 .. rb+R+2 of this lane (D-y operands)
-#pragma unroll
-            for (int fi = 0; fi < NF; fi++)
-#pragma unroll
-                for (int k = 0; k < R + 7; k++) ywin[fi][k] = OY(sr, fi, k);
-            // L of every row (the rings are padded, so a non-output row reads its neighbours'
-            // slots harmlessly); inside the region D-x + D-y + D-z, else 0
+            // L of every row (the rings are padded, so a non-output row of a wave with output
+            // rows reads its neighbours' slots harmlessly); inside the region D-x + D-y + D-z,
+            // else 0.  Annulus-only lanes and waves (the last tile's upper halo) read no ring.
             float Lr[R][NF];
 #pragma unroll
-            for (int i = 0; i < R; i++) {
-                const bool upd = xU && yU[i] && zU && !(FST && z == 0);
+            for (int i = 0; i < R; i++)
 #pragma unroll
-                for (int fi = 0; fi < NF; fi++) {
-                    float a[8];
+                for (int fi = 0; fi < NF; fi++) Lr[i][fi] = 0.0f;
+            if (anyOut && zOut) {
+                float ywin[NF][R + 7];  // ring rows rb-4 .. rb+R+2 of this lane (D-y operands)
 #pragma unroll
-                    for (int j = 0; j < 8; j++) a[j] = oxq[fi][i][qx(j)];  // planes x-4 .. x+3
-                    const float dxm = g.idx * sten8(a);
+                for (int fi = 0; fi < NF; fi++)
 #pragma unroll
-                    for (int j = 0; j < 8; j++) a[j] = ywin[fi][i + j];  // rows y-4 .. y+3
-                    const float dym = g.idy * sten8(a);
-                    if (FST) {
+                    for (int k = 0; k < R + 7; k++) ywin[fi][k] = OY(sr, fi, k);
 #pragma unroll
-                        for (int j = 0; j < 8; j++) {
-                            // even mirror: t[-j] = t[j-1]; kz = zz < 0 reads kz' = -zz-1
-                            const int zz = z - 4 + j;
-                            const uint32_t o = 4u * (uint32_t)((sr * NF + fi) * SOZ + i * ZW);
-                            a[j] = *lds_at(zz >= 0 ? soz_l + o + 4u * j : smir_l + o + 4u * (7 - j));
+                for (int i = 0; i < R; i++) {
+                    const bool upd = xU && yU[i] && zU && !(FST && z == 0);
+#pragma unroll
+                    for (int fi = 0; fi < NF; fi++) {
+                        float a[8];
+#pragma unroll
+                        for (int j = 0; j < 8; j++) a[j] = oxq[fi][i][qx(j)];  // planes x-4 .. x+3
+                        const float dxm = g.idx * sten8(a);
+#pragma unroll
+                        for (int j = 0; j < 8; j++) a[j] = ywin[fi][i + j];  // rows y-4 .. y+3
+                        const float dym = g.idy * sten8(a);
+                        if (FST) {
+#pragma unroll
+                            for (int j = 0; j < 8; j++) {
+                                // even mirror: t[-j] = t[j-1]; kz = zz < 0 reads kz' = -zz-1
+                                const int zz = z - 4 + j;
+                                const uint32_t o = 4u * (uint32_t)((sr * NF + fi) * SOZ + i * ZW);
+                                a[j] = *lds_at(zz >= 0 ? soz_l + o + 4u * j : smir_l + o + 4u * (7 - j));
+                            }
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < 8; j++) a[j] = OZ(sr, fi, i, j);  // z-4 .. z+3
                         }
-                    } else {
-#pragma unroll
-                        for (int j = 0; j < 8; j++) a[j] = OZ(sr, fi, i, j);  // z-4 .. z+3
+                        const float dzm = g.idz * sten8(a);
+                        Lr[i][fi] = upd ? dxm + dym + dzm : 0.0f;
                     }
-                    const float dzm = g.idz * sten8(a);
-                    Lr[i][fi] = upd ? dxm + dym + dzm : 0.0f;
                 }
             }
 #pragma unroll
             for (int i = 0; i < R; i++) {
-                if (!rOut[i]) continue;
+                if (!rW[i]) continue;
                 const uint32_t so = ooff_s[i];
                 const bool upd = xU && yU[i] && zU;
                 float L[NF];
