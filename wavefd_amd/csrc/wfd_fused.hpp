@@ -41,7 +41,9 @@ template <int PHYS>
 struct FusedArgs {
     Grid g;
     float dt2;
-    int cx;        // x-planes per chunk
+    int cx;        // x-planes per chunk: chunks [0, nlong) hold cx + cxu planes, the last one up to nx
+    int nlong;
+    int cxu;       // the kernel's x-loop unrolling (its steps come in multiples of cxu)
     int nzt, nyt;  // tiles along z and y
     int nxc;       // x-chunks
     int bz, by;    // tiles per block of the XCD-local order (bz = 0: plain z-fastest order)
@@ -162,8 +164,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<fu
     const int t = threadIdx.x, l = t % ZW, r = t / ZW;
     const int z0 = zt * ZO, y0 = yt * YO;
     const int z = z0 - 4 + l, y = y0 - 4 + r;
-    const int xs = xc * A.cx;
-    const int xe = xs + A.cx < nx ? xs + A.cx : nx;
+    int xs, xe;
+    chunk_range(A, xc, xs, xe);
     const int zlo = fst ? 0 : 4;
     const bool yzU = y >= 4 && y < ny - 4 && z >= zlo && z < nz - 4;
     // outputs: the inner (ZW - 7) x (NROW - 7) cells; the last tile along an axis also writes the
@@ -488,6 +490,13 @@ __device__ __forceinline__ void tile_coords(const FusedArgs<PHYS> &A, int tile, 
     }
 }
 
+// x-planes [xs, xe) of chunk xc
+template <int PHYS>
+__device__ __forceinline__ void chunk_range(const FusedArgs<PHYS> &A, int xc, int &xs, int &xe) {
+    xs = xc * A.cx + A.cxu * min(xc, A.nlong);
+    xe = xc == A.nxc - 1 ? (int)A.g.nx : xs + A.cx + (xc < A.nlong ? A.cxu : 0);
+}
+
 // workgroup bid of a launch of nwg workgroups over A's tiles -> (zt, yt, xc); false for the
 // padding workgroups of a batch (nwg rounded up to a multiple of 8).  XCD-aware numbering:
 // workgroup id bid runs on XCD bid mod 8; each XCD owns a contiguous run of the tile order, so
@@ -573,6 +582,27 @@ __global__ void __launch_bounds__(ZW *NROW) k_fused_batch(FusedBatch<PHYS> B) {
 
 // x-planes per chunk: balance whole rounds of workgroups (per_cu per CU) against the 7-step
 // warm-up and the padding to a multiple of `unroll` steps that each chunk pays
+// nx planes in nxc chunks for a kernel whose x-loop runs in steps of u (7 warm-up steps, then one
+// step per plane, padded to a multiple of u): lengths of the form k u + 1 need no padding, so every
+// chunk gets the same such length b, nlong of them u more, and the last one the remainder.
+// 800 planes in 8 chunks of an 8-step loop: 5 x 97 + 3 x 105 = 856 steps, against 8 x 112 = 896
+// for 8 x 100
+static inline void chunk_split(long nx, long nxc, int u, int &b, int &nlong) {
+    const long per = nx / nxc;
+    const long bb = per >= 1 ? (per - 1) / u * u + 1 : 1;
+    b = (int)bb;
+    nlong = (int)std::min(nxc - 1, (nx - nxc * bb) / u);
+}
+
+// steps of the x-loop of the longest chunk of that split
+static inline long chunk_steps(long nx, long nxc, int u) {
+    int b, nlong;
+    chunk_split(nx, nxc, u, b, nlong);
+    const long last = nx - (nxc - 1) * (long)b - (long)u * nlong;
+    const long longest = std::max<long>(nlong > 0 ? b + u : b, last);
+    return (longest + 7 + u - 1) / u * u;
+}
+
 static inline int fused_cx(const Prop *p, long tiles, int unroll = 4, int per_cu = 1) {
     if (p->fused_chunk > 0) return (int)std::min<long>(p->fused_chunk, p->nx);
     if (const char *e = std::getenv("WAVEFD_FUSED_CX")) {
@@ -584,7 +614,8 @@ static inline int fused_cx(const Prop *p, long tiles, int unroll = 4, int per_cu
     double best = 1e300;
     for (long nxc = 1; nxc <= 64 && nxc <= p->nx; nxc++) {
         const long cx = (p->nx + nxc - 1) / nxc;
-        const long steps = (cx + 7 + unroll - 1) / unroll * unroll;
+        if ((p->nx + cx - 1) / cx != nxc) continue;  // no length gives nxc chunks
+        const long steps = chunk_steps(p->nx, nxc, unroll);
         const long rounds = (tiles * nxc + cus - 1) / cus;
         const double cost = (double)rounds * (double)steps;
         if (cost < best * 0.999) {
@@ -639,6 +670,8 @@ static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float 
         if (std::atoi(e) == 0) A.bz = 0;
     const long nxc = (p->nx + A.cx - 1) / A.cx;
     A.nxc = (int)nxc;
+    A.cxu = unroll;
+    chunk_split(p->nx, nxc, unroll, A.cx, A.nlong);
     A.pCur = f[F_PCUR].d;
     A.pOld = f[F_POLD].d;
     A.pSpace = linear ? nullptr : f[F_PSPACE].d;

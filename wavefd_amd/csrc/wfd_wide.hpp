@@ -83,8 +83,8 @@ __device__ __forceinline__ void wide_body(const FusedArgs<PHYS> &A, WideLds<fuse
     const int z0 = zt * ZO, y0 = yt * YO;
     const int z = z0 - 4 + l;
     const int yb = y0 - 4 + rb;
-    const int xs = xc * A.cx;
-    const int xe = xs + A.cx < nx ? xs + A.cx : nx;
+    int xs, xe;
+    chunk_range(A, xc, xs, xe);
     const int zlo = FST ? 0 : 4;
     const bool zU = z >= zlo && z < nz - 4;
     const bool zOut = l >= 4 && l < 4 + ZO && z < nz;
