@@ -473,13 +473,14 @@ def run_born(physics, nshots=24, nt=200, seed=4, big=(600, 1000, 1000), fused=Tr
 PREDICTIVE = (48, 3, 16)
 
 
-def run_store(physics="vti", n=512, nt=400, seed=5, k=96):
+def run_store(physics="vti", n=512, nt=400, seed=5, k=96, levels=(5, 11)):
     """The Born gradient with compressed snapshots judged against exact storage: one shot on an
     n^3 VTI model (the configs' construction) at 12.5 m, dt 1 ms, nt steps, a +-2 % smoothed dv;
     the image with the wave-predicted store (born.py::_WaveStore, segments of k frames, 16-bit
-    keyframes) against the image with exact snapshots (the checkpointed store, which gives the exact
-    store's image bit for bit).  Returns the stored-byte ratio against float32, the image's relative
-    L2 error and both wall times."""
+    keyframes, residuals quantised to each of `levels`) against the image with exact snapshots (the
+    checkpointed store, which gives the exact store's image bit for bit).  Returns the stored-byte
+    ratio against float32, the image's relative L2 error and the wall times, for the first of
+    `levels` and, under "variants", for every one."""
     import torch
     from wavefd_amd import born, ginsu
 
@@ -503,31 +504,36 @@ def run_store(physics="vti", n=512, nt=400, seed=5, k=96):
     wav = np.asarray(born.spacetime.causal_ricker(15.0, dt * np.arange(nt)), np.float32)
     shot = born.BornShot((20.0, n * d / 2, n * d / 2), np.full(ry.size, 20.0), ry.ravel(), rx.ravel(), wav)
     fields = {k_: b.ptr for k_, b in bufs.items()}
-    imgs, secs, stored = {}, {}, None
-    for mode in ("exact", "wave"):
+    imgs, secs, stored = {}, {}, {}
+    for mode in ("exact",) + tuple(levels):
         img = ginsu.DeviceBuffer(n ** 3 * 4)
         img.zero()
         t0 = time.perf_counter()
         r = born.born_shot(physics, win, (d, d, d), dt, shot, fields, dvb.ptr, img.ptr, nsponge=40,
                            keep_born_data=False, checkpoint=20 if mode == "exact" else None,
-                           predictive=(k, "wave", 16) if mode == "wave" else None)
+                           predictive=None if mode == "exact" else (k, "wave", 16, mode))
         img.sync()
         secs[mode] = time.perf_counter() - t0
-        if mode == "wave":
-            stored = r["stored_bytes"]
+        if mode != "exact":
+            stored[mode] = r["stored_bytes"]
         imgs[mode] = img.download(np.zeros((n, n, n), np.float32)).astype(np.float64)
         img.free()
     for b in list(bufs.values()) + [dvb]:
         b.free()
     nf = 1 if physics == "iso" else 2
-    rel = float(np.linalg.norm(imgs["wave"] - imgs["exact"]) / np.linalg.norm(imgs["exact"]))
+    variants = []
+    for lev in levels:
+        rel = float(np.linalg.norm(imgs[lev] - imgs["exact"]) / np.linalg.norm(imgs["exact"]))
+        variants.append({"residual_levels": lev, "stored_ratio_vs_float32": round(4.0 * nf * n ** 3 * nt / stored[lev], 3),
+                         "image_rel_l2_vs_exact": rel, "seconds": round(secs[lev], 2)})
     return {"workload": "%s %d^3 at 12.5 m, one Born shot (forward, Born forward, adjoint imaging), nt %d"
                         % (physics.upper(), n, nt),
-            "store": "wave-predicted (one propagator step + the source's image as predictor, quantised "
-                     "residuals, 16-bit keyframes, segments of %d frames)" % k,
-            "stored_ratio_vs_float32": round(4.0 * nf * n ** 3 * nt / stored, 3),
-            "image_rel_l2_vs_exact": rel, "seconds": {"exact_checkpointed": round(secs["exact"], 2),
-                                                      "wave_store": round(secs["wave"], 2)},
+            "store": "wave-predicted (one propagator step + the source's image as predictor, residuals quantised "
+                     "to %d levels, 16-bit keyframes, segments of %d frames)" % (levels[0], k),
+            "stored_ratio_vs_float32": variants[0]["stored_ratio_vs_float32"],
+            "image_rel_l2_vs_exact": variants[0]["image_rel_l2_vs_exact"],
+            "seconds": {"exact_checkpointed": round(secs["exact"], 2), "wave_store": variants[0]["seconds"]},
+            "variants": variants,
             "target": ">= 8x fewer snapshot bytes with the Born gradient within rel L2 1e-3 of exact storage"}
 
 

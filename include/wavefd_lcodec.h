@@ -33,8 +33,15 @@ long wavefd_lcodec_encode(const float *x, long n1, long n2, long n3, uint32_t *c
 long wavefd_lcodec_dequantize(const uint32_t *code, float *out, void *stream);  /* plain decode */
 /* the plain decode of a code of an n1 x n2 x n3 field, without reading its header (no host wait) */
 long wavefd_lcodec_dequantize_n(const uint32_t *code, long n1, long n2, long n3, float *out, void *stream);
+/* the same quantiser with levels = 11, 5 or 3 (q = rint((levels - 1) x / (2 s)); 9, 13 or 20 digits
+ * per word: 3.56, 2.5 or 1.63 bits per value); wavefd_lcodec_dequantize and _halfstep read the
+ * levels from the code's header */
+long wavefd_lcodec_code_words_l(long n1, long n2, long n3, long levels);
+long wavefd_lcodec_encode_l(const float *x, long n1, long n2, long n3, long levels, uint32_t *code, void *stream);
+long wavefd_lcodec_dequantize_nl(const uint32_t *code, long n1, long n2, long n3, long levels, float *out,
+                                 void *stream);
 float wavefd_lcodec_norm(const uint32_t *code);  /* the code's largest block scale; -1 on error */
-/* half a quantisation step of every value's block, s / 10: |x - x_hat| is at most that */
+/* half a quantisation step of every value's block, s / (levels - 1): |x - x_hat| is at most that */
 long wavefd_lcodec_halfstep(const uint32_t *code, float *out, void *stream);
 /* out = ca a + cb b + cc c over n floats (device; b and c may be NULL), evaluated in that order:
  * the predictions, residuals and reconstructions of the Born driver's predictive snapshot store,
@@ -67,6 +74,9 @@ long wavefd_lcodec_adam(void *net, float lr, void *stream);  /* one Adam step, b
  * bits where it is closer to x than the plain dequantisation (scratch1/2: the field's size) */
 long wavefd_lcodec_encode_learned(void *net, const float *x, long n1, long n2, long n3, uint32_t *code, float *scratch1,
                                   float *scratch2, void *stream);
+/* the same with levels = 11, 5 or 3 (wavefd_lcodec_encode_l's quantiser) */
+long wavefd_lcodec_encode_learned_l(void *net, const float *x, long n1, long n2, long n3, long levels, uint32_t *code,
+                                    float *scratch1, float *scratch2, void *stream);
 /* learned decode of a code (scratch: a device array of the field's size); blocks whose bit is
  * clear take the plain dequantisation */
 long wavefd_lcodec_decode(void *net, const uint32_t *code, float *scratch, float *out, void *stream);

@@ -5,7 +5,8 @@ snapshot store is CvxCompress or exact storage, src/compressedio.jl:1-148); this
 its kernels are compared against:
 
 * ``quantize`` / ``dequantize``: 8^3 blocks, scale = block max |x| rounded up to bfloat16,
-  q = rint(5 x / s) clipped to -5..5, x_hat = s (d - 5) / 5 in float32 -- bit-exact with the GPU;
+  q = rint(L x / s) clipped to -L..L, x_hat = s (d - L) / L in float32, L = (levels - 1) / 2 for
+  levels 11 (the default), 5 or 3 -- bit-exact with the GPU;
 * ``forward`` / ``loss_grad``: the decoder network out = q + (F(q) - F(-q)) / 2 with
   F = conv3(relu(conv2(relu(conv1(.))))), 3x3x3 "same" convolutions without bias (zero outside
   the volume), 16 channels; with ``bf16=True`` the
@@ -44,9 +45,13 @@ def bf16_up(x):
     return u.view(np.float32)
 
 
-def quantize(field):
+DIGITS = {11: 9, 5: 13, 3: 20}  # base-levels digits per 32-bit word: 11^9, 5^13, 3^20 < 2^32
+
+
+def quantize(field, levels=11):
     """field: (n3, n2, n1) float32 (n1 fastest).  Returns (scales (m3, m2, m1) float32,
-    digits (n3, n2, n1) int in 0..10)"""
+    digits (n3, n2, n1) int in 0..levels - 1)"""
+    LV = (levels - 1) // 2
     n3, n2, n1 = field.shape
     m3, m2, m1 = (-(-n3 // BS), -(-n2 // BS), -(-n1 // BS))
     pad = np.zeros((m3 * BS, m2 * BS, m1 * BS), np.float32)
@@ -61,16 +66,18 @@ def quantize(field):
     return s, q.reshape(m3 * BS, m2 * BS, m1 * BS)[:n3, :n2, :n1]
 
 
-def dequantize(scales, digits):
+def dequantize(scales, digits, levels=11):
+    LV = (levels - 1) // 2
     n3, n2, n1 = digits.shape
     s = np.repeat(np.repeat(np.repeat(scales, BS, 0), BS, 1), BS, 2)[:n3, :n2, :n1].astype(np.float32)
     return (s * (digits - LV).astype(np.float32) / np.float32(LV)).astype(np.float32)
 
 
-def code_words(n1, n2, n3):
-    """header, bfloat16 scales (2 per word), one choice bit per block, 57 payload words per block"""
+def code_words(n1, n2, n3, levels=11):
+    """header, bfloat16 scales (2 per word), one choice bit per block, then per block 512 digits
+    packed DIGITS[levels] to a word (57, 40 or 26 words)"""
     nb = (-(-n1 // BS)) * (-(-n2 // BS)) * (-(-n3 // BS))
-    return 8 + (nb + 1) // 2 + (nb + 31) // 32 + 57 * nb
+    return 8 + (nb + 1) // 2 + (nb + 31) // 32 + -(-512 // DIGITS[levels]) * nb
 
 
 def unpack(params):

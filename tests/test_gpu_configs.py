@@ -11,7 +11,8 @@
   <J dv, d> = <dv, J^T d> <= 1e-5 on the 600^3 window (dv zero outside its interior box);
 * VTI 512^3 Born gradient with the wave-predicted snapshot store (born.py::_WaveStore) against
   exact storage (the checkpointed store, which gives the exact store's image bit for bit, as
-  tests/test_gpu_born_checkpoint.py checks): >= 8x fewer stored bytes, image relative L2 <= 1e-3.
+  tests/test_gpu_born_checkpoint.py checks): residuals quantised to 11 levels, >= 8x fewer stored
+  bytes, and to 5 levels, >= 11x; image relative L2 <= 1e-3 for both.
   nt = 400 here (the store of 3000 frames would not fit one GPU beside the exact reference run).
 
 The models follow the configs' construction (tests/helpers.py::config_model).
@@ -191,23 +192,24 @@ def test_c5_vti_512_predictive_store_against_exact():
     wav = np.asarray(born.spacetime.causal_ricker(15.0, dt * np.arange(nt)), F32)
     shot = _shot(born, win, d, wav)
     fields = {k: b.ptr for k, b in dev.items()}
-    imgs, stored = {}, None
-    for mode in ("exact", "predictive"):
+    imgs, stored = {}, {}
+    for mode in ("exact", 11, 5):
         img = ginsu.DeviceBuffer(dv.nbytes)
         img.zero()
         r = born.born_shot("vti", win, (d, d, d), dt, shot, fields, dvb.ptr, img.ptr, nsponge=40,
                            keep_born_data=False, checkpoint=20 if mode == "exact" else None,
-                           predictive=(96, "wave", 16) if mode == "predictive" else None)
-        if mode == "predictive":
-            stored = r["stored_bytes"]
+                           predictive=None if mode == "exact" else (96, "wave", 16, mode))
+        if mode != "exact":
+            stored[mode] = r["stored_bytes"]
         imgs[mode] = img.download(np.zeros(shape, F32)).astype(np.float64)
         img.free()
-    ratio = 4.0 * 2 * nx * ny * nz * nt / stored
-    rel = float(np.linalg.norm(imgs["predictive"] - imgs["exact"]) / np.linalg.norm(imgs["exact"]))
-    print(f"C5 VTI 512^3, nt {nt}: wave-predicted store {ratio:.2f}x fewer bytes than float32, image rel L2 {rel:.2e}",
-          flush=True)
     for b in list(dev.values()) + [dvb]:
         b.free()
     assert np.isfinite(imgs["exact"]).all() and np.abs(imgs["exact"]).max() > 0
-    assert ratio >= 8.0
-    assert rel <= 1e-3
+    for levels, want_ratio in ((11, 8.0), (5, 11.0)):
+        ratio = 4.0 * 2 * nx * ny * nz * nt / stored[levels]
+        rel = float(np.linalg.norm(imgs[levels] - imgs["exact"]) / np.linalg.norm(imgs["exact"]))
+        print(f"C5 VTI 512^3, nt {nt}: wave-predicted store, residuals to {levels} levels: {ratio:.2f}x fewer bytes "
+              f"than float32, image rel L2 {rel:.2e}", flush=True)
+        assert ratio >= want_ratio
+        assert rel <= 1e-3

@@ -32,7 +32,8 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
 
-def test_quantiser_matches_oracle():
+@pytest.mark.parametrize("levels", [11, 5, 3])
+def test_quantiser_matches_oracle(levels):
     from wavefd_amd import lcodec
     rng = np.random.default_rng(0)
     n3, n2, n1 = 21, 29, 37  # (nx, ny, nz): the last block along every axis is cut
@@ -41,16 +42,22 @@ def test_quantiser_matches_oracle():
     shape = (n1, n2, n3)
     xb = _dev(x)
     from wavefd_amd import ginsu
-    cb = ginsu.DeviceBuffer(4 * lcodec.code_words(shape))
-    lcodec.encode(xb.ptr, shape, cb.ptr)
-    out = ginsu.DeviceBuffer(x.nbytes)
-    lcodec.dequantize(cb.ptr, out.ptr)
+    cb = ginsu.DeviceBuffer(4 * lcodec.code_words(shape, levels))
+    lcodec.encode(xb.ptr, shape, cb.ptr, levels=levels)
+    out, out2, hs = (ginsu.DeviceBuffer(x.nbytes) for _ in range(3))
+    lcodec.dequantize(cb.ptr, out.ptr)  # levels from the code's header
+    lcodec.dequantize(cb.ptr, out2.ptr, shape=shape, levels=levels)
     got = out.download(np.zeros_like(x))
-    s, d = ref.quantize(x)
-    assert np.array_equal(got, ref.dequantize(s, d))
+    s, d = ref.quantize(x, levels)
+    want = ref.dequantize(s, d, levels)
+    assert np.array_equal(got, want)
+    assert np.array_equal(out2.download(np.zeros_like(x)), want)
     lib = lcodec._lib.load("lcodec")
     assert lib.wavefd_lcodec_norm(cb.ptr) == float(s.max())
-    for b in (xb, cb, out):
+    assert lib.wavefd_lcodec_halfstep(cb.ptr, hs.ptr, None) == 0
+    sb = np.repeat(np.repeat(np.repeat(s, 8, 0), 8, 1), 8, 2)[:n3, :n2, :n1]
+    assert np.array_equal(hs.download(np.zeros_like(x)), (sb / F32(levels - 1)).astype(F32))
+    for b in (xb, cb, out, out2, hs):
         b.free()
 
 

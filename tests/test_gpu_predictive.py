@@ -261,12 +261,13 @@ def test_dequantize_with_known_shape_equals_header_read():
         q.free()
 
 
+@pytest.mark.parametrize("levels", [11, 5])
 @pytest.mark.parametrize("kind", ["iso", "vti"])
-def test_wave_predicted_store_against_exact(kind):
+def test_wave_predicted_store_against_exact(kind, levels):
     """born.py::_WaveStore: frames predicted by one step of a propagator with the window's model
-    plus the source's image, residuals quantised; the Born image within 1e-4 of exact storage
-    (an error each frame carries is coded away with the next frame's residual), the stored bytes
-    those of 2 keyframes and k - 2 residual codes per segment."""
+    plus the source's image, residuals quantised (11 or 5 levels); the Born image within 1e-4 of
+    exact storage (an error each frame carries is coded away with the next frame's residual), the
+    stored bytes those of 2 keyframes and k - 2 residual codes per segment."""
     from wavefd_amd import born, ginsu, lcodec, spacetime
     shape = (64, 56, 60)
     nx, ny, nz = shape
@@ -284,7 +285,7 @@ def test_wave_predicted_store_against_exact(kind):
         img = ginsu.DeviceBuffer(dv.nbytes)
         img.zero()
         r = born.born_shot(kind, win, (10.0, 10.0, 10.0), 1e-3, sh, model, dvb.ptr, img.ptr, nsponge=10,
-                           predictive=(k, "wave", 16) if mode == "wave" else None)
+                           predictive=(k, "wave", 16, levels) if mode == "wave" else None)
         imgs[mode] = img.download(np.zeros(shape, F32)).astype(np.float64)
         if mode == "wave":
             stored = r["stored_bytes"]
@@ -298,8 +299,9 @@ def test_wave_predicted_store_against_exact(kind):
     cod = _lib.load("codec")
     kwords = cod.wavefd_codec_words((ctypes.c_long * 3)(nz, ny, nx), (ctypes.c_long * 3)(8, 8, 8), 16)
     nkey = 2 * (nt // k)
-    want = 4 * (kwords * nkey + lcodec.code_words((nz, ny, nx)) * (nt - nkey)) * nf
-    print(f"wave-predicted store {kind}: image rel err {rel:.3e}, {4 * nx * ny * nz * nt * nf / stored:.2f}x",
+    want = 4 * (kwords * nkey + lcodec.code_words((nz, ny, nx), levels) * (nt - nkey)) * nf
+    print(f"wave-predicted store {kind}, {levels} levels: image rel err {rel:.3e}, "
+          f"{4 * nx * ny * nz * nt * nf / stored:.2f}x",
           flush=True)
     assert stored == want
     assert np.abs(imgs["exact"]).max() > 0 and rel <= 1e-4

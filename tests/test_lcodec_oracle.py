@@ -70,23 +70,32 @@ def test_padding_carries_no_gradient():
     assert not np.any(g[ref.mask() == 0])
 
 
-def test_quantiser_bound_and_levels():
+@pytest.mark.parametrize("levels", [11, 5, 3])
+def test_quantiser_bound_and_levels(levels):
+    half = (levels - 1) // 2
     rng = np.random.default_rng(3)
     x = (rng.standard_normal((19, 13, 21)) * np.exp(rng.uniform(-8, 8, (19, 1, 1)))).astype(np.float32)
-    s, d = ref.quantize(x)
-    xh = ref.dequantize(s, d)
+    s, d = ref.quantize(x, levels)
+    assert d.min() >= 0 and d.max() <= levels - 1
+    xh = ref.dequantize(s, d, levels)
     sb = np.repeat(np.repeat(np.repeat(s, 8, 0), 8, 1), 8, 2)[:19, :13, :21]
-    assert np.all(np.abs(xh - x) <= sb / 10 * (1 + 1e-6) + 1e-30)
+    assert np.all(np.abs(xh - x) <= sb / (2 * half) * (1 + 1e-6) + 1e-30)
     assert np.all(sb >= np.abs(x))  # scales rounded up: nothing clips
     # values on the levels come back exactly
     sc = ref.bf16(rng.uniform(0.1, 10.0, (2, 2, 3)).astype(np.float32))
-    dg = rng.integers(0, 11, (16, 16, 24))
-    dg[::8, ::8, ::8] = 10  # every block reaches its scale
+    dg = rng.integers(0, levels, (16, 16, 24))
+    dg[::8, ::8, ::8] = levels - 1  # every block reaches its scale
     sb = np.repeat(np.repeat(np.repeat(sc, 8, 0), 8, 1), 8, 2)
-    lv = (sb * (dg - 5).astype(np.float32) / np.float32(5)).astype(np.float32)
-    s2, d2 = ref.quantize(lv)
+    lv = (sb * (dg - half).astype(np.float32) / np.float32(half)).astype(np.float32)
+    s2, d2 = ref.quantize(lv, levels)
     assert np.array_equal(s2, sc)
-    assert np.array_equal(ref.dequantize(s2, d2), lv)
+    assert np.array_equal(ref.dequantize(s2, d2, levels), lv)
+
+
+def test_digits_fit_a_word():
+    # the packing: DIGITS[levels] base-levels digits must fit 32 bits, one more must not
+    for levels, dg in ref.DIGITS.items():
+        assert levels ** dg < 2 ** 32 <= levels ** (dg + 1)
 
 
 def test_bf16_rounding():
@@ -97,10 +106,20 @@ def test_bf16_rounding():
     assert up[0] == np.float32(1.0 + 2 ** -7)
 
 
+@pytest.mark.parametrize("levels", [11, 5, 3])
 @pytest.mark.parametrize("shape", [(512, 512, 512), (37, 29, 21), (8, 8, 8)])
-def test_code_size(shape):
+def test_code_size(shape, levels):
     from wavefd_amd import lcodec
     nz, ny, nx = shape
-    assert lcodec.code_words(shape) == ref.code_words(nz, ny, nx)
+    assert lcodec.code_words(shape, levels) == ref.code_words(nz, ny, nx, levels)
+    if levels == 11:
+        assert lcodec.code_words(shape) == ref.code_words(nz, ny, nx)
     if shape == (512, 512, 512):
-        assert lcodec.ratio(shape) >= 8.0
+        assert lcodec.ratio(shape, levels) >= {11: 8.0, 5: 12.5, 3: 19.0}[levels]
+
+
+def test_code_size_rejects_other_levels():
+    from wavefd_amd import lcodec
+    for levels in (2, 4, 7, 13):
+        with pytest.raises(ValueError):
+            lcodec.code_words((16, 16, 16), levels)

@@ -136,6 +136,9 @@ LCODEC = {
     "wavefd_lcodec_encode": (_L, [_P, _L, _L, _L, _P, _P]),
     "wavefd_lcodec_dequantize": (_L, [_P, _P, _P]),
     "wavefd_lcodec_dequantize_n": (_L, [_P, _L, _L, _L, _P, _P]),
+    "wavefd_lcodec_code_words_l": (_L, [_L, _L, _L, _L]),
+    "wavefd_lcodec_encode_l": (_L, [_P, _L, _L, _L, _L, _P, _P]),
+    "wavefd_lcodec_dequantize_nl": (_L, [_P, _L, _L, _L, _L, _P, _P]),
     "wavefd_lcodec_nparams": (_L, []),
     "wavefd_lcodec_net_create": (_P, []),
     "wavefd_lcodec_net_free": (None, [_P]),
@@ -146,6 +149,7 @@ LCODEC = {
     "wavefd_lcodec_grad": (_L, [_P, _P, _P, _L, _L, _L, _L, ctypes.c_float, _P, _P, _P, _P, _P]),
     "wavefd_lcodec_halfstep": (_L, [_P, _P, _P]),
     "wavefd_lcodec_encode_learned": (_L, [_P, _P, _L, _L, _L, _P, _P, _P, _P]),
+    "wavefd_lcodec_encode_learned_l": (_L, [_P, _P, _L, _L, _L, _L, _P, _P, _P, _P]),
     "wavefd_lcodec_adam": (_L, [_P, ctypes.c_float, _P]),
     "wavefd_lcodec_decode": (_L, [_P, _P, _P, _P, _P]),
     "wavefd_lcodec_gather": (_L, [_P, _L, _L, _L, _P, _L, _L, ctypes.c_float, _P, _P]),

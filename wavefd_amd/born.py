@@ -448,11 +448,12 @@ class _WaveStore:
     dominates); working fields: the segment buffer (k per wavefield), the frame the step writes,
     the residual, the source image G and the second propagator."""
 
-    def __init__(self, run, k=96, keybits=16):
+    def __init__(self, run, k=96, keybits=16, levels=11, codec=None):
         from . import lcodec, _lib
-        if int(k) < 3 or int(keybits) not in (8, 16):
-            raise ValueError("wave-predicted store: segments of at least 3 frames, 8- or 16-bit keyframes")
-        self.run, self.lc, self.k, self.keybits = run, lcodec, int(k), int(keybits)
+        if int(k) < 3 or int(keybits) not in (8, 16) or int(levels) not in lcodec.LEVELS:
+            raise ValueError("wave-predicted store: segments of at least 3 frames, 8- or 16-bit keyframes, "
+                             "residuals quantised to %s levels" % (lcodec.LEVELS,))
+        self.run, self.lc, self.k, self.keybits, self.levels = run, lcodec, int(k), int(keybits), int(levels)
         self.shape, self.nt, self.nf = run.window.size, run.nt, run.nf
         self.ncell = ncell = run.ncell
         self.cod = _lib.load("codec")
@@ -460,7 +461,7 @@ class _WaveStore:
         self.n3 = (ctypes.c_long * 3)(nz, ny, nx)
         self.b3 = (ctypes.c_long * 3)(8, 8, 8)
         self.kwords = int(self.cod.wavefd_codec_words(self.n3, self.b3, self.keybits))
-        self.words = lcodec.code_words(self.shape)
+        self.words = lcodec.code_words(self.shape, self.levels)
         self.nseg = (self.nt + self.k - 1) // self.k
         self.nkey = sum(min(2, self.nt - j * self.k) for j in range(self.nseg))  # keyframes per wavefield
         self.keys = ginsu.DeviceBuffer(4 * self.kwords * self.nkey * self.nf)
@@ -469,6 +470,11 @@ class _WaveStore:
         self.tmp = ginsu.DeviceBuffer(4 * ncell)
         self.seg = ginsu.DeviceBuffer(4 * ncell * min(self.k, self.nt) * self.nf)
         self.cached = None
+        # optional learned residual decoder (lcodec.LearnedCodec trained on this store's residual
+        # frames at these levels): its encode (with the per-block choice) and decode replace the
+        # plain quantiser's on both sides, so the reconstructions still agree bit for bit
+        self.codec = codec
+        self.scr = [ginsu.DeviceBuffer(4 * ncell) for _ in range(2)] if codec is not None else []
         # the predictor: a propagator with the window's model; Q needs no copy (the same
         # construction parameters give the same profile)
         p2 = run.new_prop()
@@ -564,8 +570,13 @@ class _WaveStore:
             self._predict(n, stream)
         pred = self._ptr(self.cur[kind])
         lc.combine(nc, 1.0, self.x[kind].ptr, self.tmp.ptr, -1.0, pred, stream=stream)  # residual
-        lc.encode(self.tmp.ptr, self.shape, self._code(kind, n), stream)
-        lc.dequantize(self._code(kind, n), self.tmp.ptr, stream, self.shape)
+        if self.codec is None:
+            lc.encode(self.tmp.ptr, self.shape, self._code(kind, n), stream, self.levels)
+            lc.dequantize(self._code(kind, n), self.tmp.ptr, stream, self.shape, self.levels)
+        else:
+            self.codec.encode(self.tmp.ptr, self.shape, self._code(kind, n), self.scr[0].ptr, self.scr[1].ptr, stream,
+                              self.levels)
+            self.codec.decode(self._code(kind, n), self.tmp.ptr, self.scr[0].ptr, stream)
         lc.combine(nc, 1.0, pred, pred, 1.0, self.tmp.ptr, stream=stream)  # the reconstruction
         if kind == self.nf - 1:
             ginsu.sync(stream)
@@ -581,7 +592,10 @@ class _WaveStore:
                 if m < 2:
                     self._keyframe(kind, n, None, stream)
                 else:
-                    self.lc.dequantize(self._code(kind, n), self.tmp.ptr, stream, self.shape)
+                    if self.codec is None:
+                        self.lc.dequantize(self._code(kind, n), self.tmp.ptr, stream, self.shape, self.levels)
+                    else:
+                        self.codec.decode(self._code(kind, n), self.tmp.ptr, self.scr[0].ptr, stream)
                     self.lc.combine(self.ncell, 1.0, cur, cur, 1.0, self.tmp.ptr, stream=stream)
                 ginsu.copy(self._slot(kind, n), cur, 4 * self.ncell, stream)
             ginsu.sync(stream)
@@ -594,7 +608,7 @@ class _WaveStore:
         return self._slot(kind, n)
 
     def free(self):
-        for b in [self.keys, self.codes, self.tmp, self.seg] + self.x + self.g:
+        for b in [self.keys, self.codes, self.tmp, self.seg] + self.x + self.g + self.scr:
             b.free()
         prop3d.free(self.p2)
 
@@ -661,7 +675,7 @@ class _ShotRun:
         if checkpoint is not None:
             self.store = _CheckpointStore(self, checkpoint, fused)
         elif isinstance(predictive, tuple) and len(predictive) > 1 and predictive[1] == "wave":
-            self.store = _WaveStore(self, predictive[0], *predictive[2:])
+            self.store = _WaveStore(self, predictive[0], *predictive[2:], codec=codec_for_residuals)
         elif predictive is not None:
             self.store = _PredictiveStore(window.size, nt, nf, *(predictive if isinstance(predictive, tuple)
                                                                   else (predictive,)),

@@ -60,8 +60,7 @@ std::string g_err;
 
 constexpr uint32_t MAGIC = 0x5746444cu;  // "WFDL"
 constexpr int BS = 8, BV = BS * BS * BS;  // block extent and values per block
-constexpr int WPB = 57;                   // payload words per block (9 digits per word)
-constexpr int LV = 5;                     // levels -LV..LV
+constexpr int LV = 5;                     // levels -LV..LV of the default quantiser (11 levels)
 constexpr int CH = 16;                    // channels
 constexpr int NT = 27;                    // taps
 constexpr int LAYER_W = NT * CH * CH;     // weights per layer (padded)
@@ -88,18 +87,26 @@ __host__ __device__ inline uint16_t f2bf_up(float f) {  // smallest bfloat16 >= 
 }
 
 // ---- quantiser --------------------------------------------------------------------------
+// 2 lv + 1 levels for lv = 5 (11 levels, the default), 2 (5 levels) or 1 (3 levels): q_digits
+// base-(2 lv + 1) digits per 32-bit word (11^9, 5^13, 3^20 < 2^32), q_words words per 8^3 block
+// (57, 40, 26: 3.56, 2.5 and 1.63 bits per value)
+__host__ __device__ constexpr int q_digits(int lv) { return lv == 5 ? 9 : (lv == 2 ? 13 : 20); }
+__host__ __device__ constexpr int q_words(int lv) { return (BV + q_digits(lv) - 1) / q_digits(lv); }
+inline bool q_levels_ok(long lv) { return lv == 5 || lv == 2 || lv == 1; }
+
 struct QGeo {
     long n1, n2, n3, m1, m2, m3;
+    int lv;
     __host__ __device__ long nblocks() const { return m1 * m2 * m3; }
 };
-inline QGeo qgeo(long n1, long n2, long n3) {
-    return {n1, n2, n3, (n1 + BS - 1) / BS, (n2 + BS - 1) / BS, (n3 + BS - 1) / BS};
+inline QGeo qgeo(long n1, long n2, long n3, int lv = LV) {
+    return {n1, n2, n3, (n1 + BS - 1) / BS, (n2 + BS - 1) / BS, (n3 + BS - 1) / BS, lv};
 }
 __host__ __device__ inline long scale_words(const QGeo &g) { return (g.nblocks() + 1) / 2; }
 // one bit per block after the scales: 1 = the block's decode is the learned one
 __host__ __device__ inline long flag_off(const QGeo &g) { return 8 + scale_words(g); }
 __host__ __device__ inline long pay_off(const QGeo &g) { return flag_off(g) + (g.nblocks() + 31) / 32; }
-inline long code_words(const QGeo &g) { return pay_off(g) + WPB * g.nblocks(); }
+inline long code_words(const QGeo &g) { return pay_off(g) + (long)q_words(g.lv) * g.nblocks(); }
 
 __device__ __forceinline__ long block_elem(const QGeo &g, long blk, int e, bool &in) {
     const long k1 = blk % g.m1, k2 = (blk / g.m1) % g.m2, k3 = blk / (g.m1 * g.m2);
@@ -111,7 +118,9 @@ __device__ __forceinline__ long block_elem(const QGeo &g, long blk, int e, bool 
 // Four blocks along n1 per workgroup, the layout of k_dequantize: lane tid reads the 8 values of
 // block tid % 4 in row tid / 4 (four lanes, one 128-byte line); each block's maximum is a shuffle
 // over the 16 lanes of a wave that hold it, then over the 4 waves; one wave packs each block.
+template <int L>
 __global__ void __launch_bounds__(256) k_encode(QGeo g, const float *__restrict__ x, uint32_t *__restrict__ code) {
+    constexpr int D = q_digits(L), W = q_words(L);
     const long m1q = (g.m1 + 3) / 4;
     const long wg = blockIdx.x;
     const long k1q = wg % m1q, rest = wg / m1q, k2 = rest % g.m2, k3 = rest / g.m2;
@@ -146,34 +155,35 @@ __global__ void __launch_bounds__(256) k_encode(QGeo g, const float *__restrict_
     for (int j = 0; j < BS; j++) {
         int qv = 0;
         if (s > 0.0f) {
-            qv = (int)rintf(v[j] / s * (float)LV);
-            qv = qv < -LV ? -LV : (qv > LV ? LV : qv);
+            qv = (int)rintf(v[j] / s * (float)L);
+            qv = qv < -L ? -L : (qv > L ? L : qv);
         }
-        dig[q][r * BS + j] = (uint8_t)(qv + LV);
+        dig[q][r * BS + j] = (uint8_t)(qv + L);
     }
     __syncthreads();
     const long k1w = k1q * 4 + w;  // the block wave w packs
     if (k1w >= g.m1) return;
     const long blk = (k3 * g.m2 + k2) * g.m1 + k1w;
     if (t == 0) reinterpret_cast<uint16_t *>(code + 8)[blk] = f2bf_up(sc[w]);  // the largest goes to word 7 (k_norm)
-    if (t < WPB) {
+    if (t < W) {
         uint32_t wd = 0, p = 1;
-        for (int j = 0; j < 9; j++) {
-            const int e = t * 9 + j;
-            wd += (e < BV ? (uint32_t)dig[w][e] : (uint32_t)LV) * p;
-            p *= 11u;
+        for (int j = 0; j < D; j++) {
+            const int e = t * D + j;
+            wd += (e < BV ? (uint32_t)dig[w][e] : (uint32_t)L) * p;
+            p *= (uint32_t)(2 * L + 1);
         }
-        code[pay_off(g) + WPB * blk + t] = wd;
+        code[pay_off(g) + (long)W * blk + t] = wd;
     }
 }
 
-__device__ __forceinline__ float dequant(float s, int d) { return s * (float)(d - LV) / (float)LV; }
+template <int L>
+__device__ __forceinline__ float dequant(float s, int d) { return s * (float)(d - L) / (float)L; }
 
-// half a quantisation step of every value's block, s / 10 (the bound of |x - x_hat|)
+// half a quantisation step of every value's block, s / (2 lv) (the bound of |x - x_hat|)
 __global__ void __launch_bounds__(256) k_halfstep(QGeo g, const uint32_t *__restrict__ code, float *__restrict__ out) {
     const long blk = blockIdx.x;
     const uint16_t *scales = reinterpret_cast<const uint16_t *>(code + 8);
-    const float h = bf2f(scales[blk]) / (float)(2 * LV);
+    const float h = bf2f(scales[blk]) / (float)(2 * g.lv);
     for (int e = threadIdx.x; e < BV; e += blockDim.x) {
         bool in;
         const long k = block_elem(g, blk, e, in);
@@ -217,23 +227,25 @@ __global__ void __launch_bounds__(1024) k_norm(QGeo g, uint32_t *__restrict__ co
 // stores go row by row: lane tid writes the 8 values of block tid % 4 in row tid / 4, so four
 // consecutive lanes write 32 consecutive floats of the grid (one 128-byte line) instead of one
 // 32-byte piece per lane.
+template <int L>
 __global__ void __launch_bounds__(256) k_dequantize(QGeo g, const uint32_t *__restrict__ code, float *__restrict__ out) {
+    constexpr int D = q_digits(L), W = q_words(L);
     const long m1q = (g.m1 + 3) / 4;
     const long wg = blockIdx.x;
     const long k1q = wg % m1q, rest = wg / m1q, k2 = rest % g.m2, k3 = rest / g.m2;
     const int tid = threadIdx.x, w = tid >> 6, t = tid & 63;
-    __shared__ uint8_t dig[4][9 * WPB];
+    __shared__ uint8_t dig[4][D * W];
     __shared__ float sc[4];
     const uint16_t *scales = reinterpret_cast<const uint16_t *>(code + 8);
     const long k1w = k1q * 4 + w;  // the block wave w unpacks
     if (k1w < g.m1) {
         const long blk = (k3 * g.m2 + k2) * g.m1 + k1w;
         if (t == 0) sc[w] = bf2f(scales[blk]);
-        if (t < WPB) {
-            uint32_t word = code[pay_off(g) + WPB * blk + t];
-            for (int j = 0; j < 9; j++) {
-                dig[w][t * 9 + j] = (uint8_t)(word % 11u);
-                word /= 11u;
+        if (t < W) {
+            uint32_t word = code[pay_off(g) + (long)W * blk + t];
+            for (int j = 0; j < D; j++) {
+                dig[w][t * D + j] = (uint8_t)(word % (uint32_t)(2 * L + 1));
+                word /= (uint32_t)(2 * L + 1);
             }
         }
     }
@@ -247,10 +259,19 @@ __global__ void __launch_bounds__(256) k_dequantize(QGeo g, const uint32_t *__re
     const float s = sc[q];
 #pragma unroll
     for (int j = 0; j < BS; j++)
-        if (j < nrow) out[base + j] = dequant(s, dig[q][r * BS + j]);
+        if (j < nrow) out[base + j] = dequant<L>(s, dig[q][r * BS + j]);
 }
 
 static inline unsigned dequantize_grid(const QGeo &g) { return (unsigned)(((g.m1 + 3) / 4) * g.m2 * g.m3); }
+
+static inline void launch_dequantize(const QGeo &g, const uint32_t *code, float *out, hipStream_t s) {
+    if (g.lv == 1)
+        k_dequantize<1><<<dequantize_grid(g), 256, 0, s>>>(g, code, out);
+    else if (g.lv == 2)
+        k_dequantize<2><<<dequantize_grid(g), 256, 0, s>>>(g, code, out);
+    else
+        k_dequantize<5><<<dequantize_grid(g), 256, 0, s>>>(g, code, out);
+}
 
 // encoder's choice per block: flag the block when the learned decode o is closer to the exact x
 // than the plain dequantisation qh (sums of squares over the block's values)
@@ -721,8 +742,8 @@ __global__ void k_lincomb(long n, float ca, const float *__restrict__ a, float c
 // the code's header (magic, grid, block size, levels, digits per word; word 7, the largest
 // block scale, is written by k_norm after the blocks), written on the stream so that encoding
 // needs no host memory and no wait
-__global__ void k_header(uint32_t *code, uint32_t n1, uint32_t n2, uint32_t n3) {
-    const uint32_t h[8] = {MAGIC, n1, n2, n3, (uint32_t)BS, (uint32_t)LV, 9u, 0u};
+__global__ void k_header(uint32_t *code, uint32_t n1, uint32_t n2, uint32_t n3, uint32_t lv) {
+    const uint32_t h[8] = {MAGIC, n1, n2, n3, (uint32_t)BS, lv, (uint32_t)q_digits((int)lv), 0u};
     if (threadIdx.x < 8) code[threadIdx.x] = h[threadIdx.x];
 }
 
@@ -760,35 +781,53 @@ long wavefd_lcodec_code_words(long n1, long n2, long n3) {
     return code_words(qgeo(n1, n2, n3));
 }
 
-long wavefd_lcodec_encode(const float *x, long n1, long n2, long n3, uint32_t *code, void *stream) {
-    if (n1 <= 0 || n2 <= 0 || n3 <= 0 || !x || !code) {
-        g_err = "encode: bad arguments";
+// levels: 11 (the default), 5 or 3 quantisation levels
+long wavefd_lcodec_code_words_l(long n1, long n2, long n3, long levels) {
+    if (n1 <= 0 || n2 <= 0 || n3 <= 0 || !q_levels_ok((levels - 1) / 2) || levels % 2 == 0) return -1;
+    return code_words(qgeo(n1, n2, n3, (int)((levels - 1) / 2)));
+}
+
+long wavefd_lcodec_encode_l(const float *x, long n1, long n2, long n3, long levels, uint32_t *code, void *stream) {
+    if (n1 <= 0 || n2 <= 0 || n3 <= 0 || !x || !code || levels % 2 == 0 || !q_levels_ok((levels - 1) / 2)) {
+        g_err = "encode: bad arguments (levels 11, 5 or 3)";
         return -1;
     }
-    const QGeo g = qgeo(n1, n2, n3);
+    const QGeo g = qgeo(n1, n2, n3, (int)((levels - 1) / 2));
     hipStream_t s = (hipStream_t)stream;
-    k_header<<<1, 64, 0, s>>>(code, (uint32_t)n1, (uint32_t)n2, (uint32_t)n3);
+    k_header<<<1, 64, 0, s>>>(code, (uint32_t)n1, (uint32_t)n2, (uint32_t)n3, (uint32_t)g.lv);
     LC_TRY(hipGetLastError());
     LC_TRY(hipMemsetAsync(code + 8, 0, sizeof(uint32_t) * (pay_off(g) - 8), s));  // scales and flags
-    k_encode<<<(unsigned)(((g.m1 + 3) / 4) * g.m2 * g.m3), 256, 0, s>>>(g, x, code);
+    const unsigned nwg = (unsigned)(((g.m1 + 3) / 4) * g.m2 * g.m3);
+    if (g.lv == 1)
+        k_encode<1><<<nwg, 256, 0, s>>>(g, x, code);
+    else if (g.lv == 2)
+        k_encode<2><<<nwg, 256, 0, s>>>(g, x, code);
+    else
+        k_encode<5><<<nwg, 256, 0, s>>>(g, x, code);
     LC_TRY(hipGetLastError());
     k_norm<<<1, 1024, 0, s>>>(g, code);
     LC_TRY(hipGetLastError());
     return 0;
 }
 
+long wavefd_lcodec_encode(const float *x, long n1, long n2, long n3, uint32_t *code, void *stream) {
+    return wavefd_lcodec_encode_l(x, n1, n2, n3, 2 * LV + 1, code, stream);
+}
+
 // the grid dimensions stored in a code (device memory)
-static long read_header(const uint32_t *code, long *n, hipStream_t s, float *norm = nullptr) {
+static long read_header(const uint32_t *code, long *n, hipStream_t s, float *norm = nullptr, int *lv = nullptr) {
     uint32_t hdr[8];
     LC_TRY(hipMemcpyAsync(hdr, code, sizeof(hdr), hipMemcpyDeviceToHost, s));
     LC_TRY(hipStreamSynchronize(s));
-    if (hdr[0] != MAGIC || hdr[4] != (uint32_t)BS || hdr[5] != (uint32_t)LV) {
+    if (hdr[0] != MAGIC || hdr[4] != (uint32_t)BS || !q_levels_ok((long)hdr[5]) ||
+        hdr[6] != (uint32_t)q_digits((int)hdr[5])) {
         g_err = "not a learned-codec code";
         return -1;
     }
     n[0] = hdr[1];
     n[1] = hdr[2];
     n[2] = hdr[3];
+    if (lv) *lv = (int)hdr[5];
     if (norm) {
         float g;
         memcpy(&g, &hdr[7], 4);
@@ -808,23 +847,27 @@ float wavefd_lcodec_norm(const uint32_t *code) {
 long wavefd_lcodec_dequantize(const uint32_t *code, float *out, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     long n[3];
-    if (read_header(code, n, s)) return -1;
-    const QGeo g = qgeo(n[0], n[1], n[2]);
-    k_dequantize<<<dequantize_grid(g), 256, 0, s>>>(g, code, out);
+    int lv = LV;
+    if (read_header(code, n, s, nullptr, &lv)) return -1;
+    launch_dequantize(qgeo(n[0], n[1], n[2], lv), code, out, s);
     LC_TRY(hipGetLastError());
     return 0;
 }
 
 // the plain decode of a code whose grid the caller knows (no header read, no host wait)
-long wavefd_lcodec_dequantize_n(const uint32_t *code, long n1, long n2, long n3, float *out, void *stream) {
-    if (n1 <= 0 || n2 <= 0 || n3 <= 0 || !code || !out) {
-        g_err = "dequantize_n: bad arguments";
+long wavefd_lcodec_dequantize_nl(const uint32_t *code, long n1, long n2, long n3, long levels, float *out,
+                                 void *stream) {
+    if (n1 <= 0 || n2 <= 0 || n3 <= 0 || !code || !out || levels % 2 == 0 || !q_levels_ok((levels - 1) / 2)) {
+        g_err = "dequantize_n: bad arguments (levels 11, 5 or 3)";
         return -1;
     }
-    const QGeo g = qgeo(n1, n2, n3);
-    k_dequantize<<<dequantize_grid(g), 256, 0, (hipStream_t)stream>>>(g, code, out);
+    launch_dequantize(qgeo(n1, n2, n3, (int)((levels - 1) / 2)), code, out, (hipStream_t)stream);
     LC_TRY(hipGetLastError());
     return 0;
+}
+
+long wavefd_lcodec_dequantize_n(const uint32_t *code, long n1, long n2, long n3, float *out, void *stream) {
+    return wavefd_lcodec_dequantize_nl(code, n1, n2, n3, 2 * LV + 1, out, stream);
 }
 
 // pred = ca a + cb b + cc c and res = x - pred over n floats (device; b, c may be NULL)
@@ -841,12 +884,13 @@ long wavefd_lcodec_predict_residual(long n, float ca, const float *a, float cb, 
     return 0;
 }
 
-// half a quantisation step for every value of the field, s / 10 (the bound of |x - x_hat|)
+// half a quantisation step for every value of the field, s / (levels - 1) (the bound of |x - x_hat|)
 long wavefd_lcodec_halfstep(const uint32_t *code, float *out, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     long n[3];
-    if (read_header(code, n, s)) return -1;
-    const QGeo g = qgeo(n[0], n[1], n[2]);
+    int lv = LV;
+    if (read_header(code, n, s, nullptr, &lv)) return -1;
+    const QGeo g = qgeo(n[0], n[1], n[2], lv);
     k_halfstep<<<(unsigned)g.nblocks(), 256, 0, s>>>(g, code, out);
     LC_TRY(hipGetLastError());
     return 0;
@@ -980,11 +1024,11 @@ long wavefd_lcodec_decode(void *net, const uint32_t *code, float *scratch, float
 // encode with the encoder's per-block choice: quantise, run the learned decode, and flag the
 // blocks where it is closer to x than the plain dequantisation (so that no block's decode is
 // worse than plain); scratch1, scratch2: device arrays of the field's size
-long wavefd_lcodec_encode_learned(void *net, const float *x, long n1, long n2, long n3, uint32_t *code, float *scratch1,
-                                  float *scratch2, void *stream) {
+long wavefd_lcodec_encode_learned_l(void *net, const float *x, long n1, long n2, long n3, long levels, uint32_t *code,
+                                    float *scratch1, float *scratch2, void *stream) {
     Net *n = reinterpret_cast<Net *>(net);
     hipStream_t s = (hipStream_t)stream;
-    if (wavefd_lcodec_encode(x, n1, n2, n3, code, stream)) return -1;
+    if (wavefd_lcodec_encode_l(x, n1, n2, n3, levels, code, stream)) return -1;
     float norm = 1.0f;
     long d[3];
     if (read_header(code, d, s, &norm)) return -1;
@@ -995,6 +1039,11 @@ long wavefd_lcodec_encode_learned(void *net, const float *x, long n1, long n2, l
     k_select<<<(unsigned)g.nblocks(), 256, 0, s>>>(g, x, scratch1, scratch2, code);
     LC_TRY(hipGetLastError());
     return 0;
+}
+
+long wavefd_lcodec_encode_learned(void *net, const float *x, long n1, long n2, long n3, uint32_t *code, float *scratch1,
+                                  float *scratch2, void *stream) {
+    return wavefd_lcodec_encode_learned_l(net, x, n1, n2, n3, 2 * LV + 1, code, scratch1, scratch2, stream);
 }
 
 long wavefd_lcodec_gather(const float *src, long n1, long n2, long n3, const long *origins, long npatch, long P,

@@ -20,33 +20,41 @@ def _err():
     return _lib.last_error("lcodec")
 
 
-def code_words(shape):
+LEVELS = (11, 5, 3)  # quantiser levels: 3.56, 2.5 or 1.63 bits per value
+
+
+def code_words(shape, levels=11):
     """32-bit words of the code of a (nz, ny, nx) field"""
     nz, ny, nx = shape
-    return int(_lib.load("lcodec").wavefd_lcodec_code_words(nz, ny, nx))
+    n = int(_lib.load("lcodec").wavefd_lcodec_code_words_l(nz, ny, nx, int(levels)))
+    if n < 0:
+        raise ValueError("quantiser levels: one of %s" % (LEVELS,))
+    return n
 
 
-def ratio(shape):
+def ratio(shape, levels=11):
     """float32 bytes / code bytes"""
     nz, ny, nx = shape
-    return nz * ny * nx / code_words(shape)
+    return nz * ny * nx / code_words(shape, levels)
 
 
-def encode(field, shape, code, stream=None):
-    """quantise the device field (nz, ny, nx) into the device buffer code (code_words words)"""
+def encode(field, shape, code, stream=None, levels=11):
+    """quantise the device field (nz, ny, nx) into the device buffer code (code_words words):
+    8^3 blocks, a bfloat16 scale per block (its largest |x| rounded up), q = rint((levels - 1) x /
+    (2 s))"""
     nz, ny, nx = shape
-    if _lib.load("lcodec").wavefd_lcodec_encode(int(field), nz, ny, nx, int(code), stream) != 0:
+    if _lib.load("lcodec").wavefd_lcodec_encode_l(int(field), nz, ny, nx, int(levels), int(code), stream) != 0:
         raise RuntimeError(_err())
 
 
-def dequantize(code, out, stream=None, shape=None):
-    """the plain decode (no network): out = s (d - 5) / 5; with the field's (nz, ny, nx) shape
-    given, without reading the code's header (no host wait)"""
+def dequantize(code, out, stream=None, shape=None, levels=11):
+    """the plain decode (no network): out = 2 s d / (levels - 1); with the field's (nz, ny, nx)
+    shape (and the code's levels) given, without reading the code's header (no host wait)"""
     lib = _lib.load("lcodec")
     if shape is None:
         rc = lib.wavefd_lcodec_dequantize(int(code), int(out), stream)
     else:
-        rc = lib.wavefd_lcodec_dequantize_n(int(code), *[int(v) for v in shape], int(out), stream)
+        rc = lib.wavefd_lcodec_dequantize_nl(int(code), *[int(v) for v in shape], int(levels), int(out), stream)
     if rc != 0:
         raise RuntimeError(_err())
 
@@ -146,12 +154,12 @@ class LearnedCodec:
             raise RuntimeError(_err())
 
     # ---- codec ----------------------------------------------------------------------------
-    def encode(self, field, shape, code, scratch1, scratch2, stream=None):
+    def encode(self, field, shape, code, scratch1, scratch2, stream=None, levels=11):
         """quantise and record, per 8^3 block, whether the learned decode beats the plain one
         (scratch1, scratch2: device arrays of the field's size)"""
         nz, ny, nx = shape
-        if self.lib.wavefd_lcodec_encode_learned(self.net, int(field), nz, ny, nx, int(code), int(scratch1),
-                                                 int(scratch2), stream) != 0:
+        if self.lib.wavefd_lcodec_encode_learned_l(self.net, int(field), nz, ny, nx, int(levels), int(code),
+                                                   int(scratch1), int(scratch2), stream) != 0:
             raise RuntimeError(_err())
 
     def decode(self, code, out, scratch, stream=None):
@@ -160,7 +168,7 @@ class LearnedCodec:
         if self.lib.wavefd_lcodec_decode(self.net, int(code), int(scratch), int(out), stream) != 0:
             raise RuntimeError(_err())
 
-    def train(self, snapshots, shape, iters=1000, patch=32, batch=4, lr=1e-3, seed=0, log_every=0):
+    def train(self, snapshots, shape, iters=1000, patch=32, batch=4, lr=1e-3, seed=0, log_every=0, levels=11):
         """Train the decoder on device snapshots (addresses of (nz, ny, nx) float32 fields): each
         step takes `batch` patch^3 cubes of random snapshots (half of them drawn where the energy
         is, half uniformly), their quantised-dequantised
@@ -170,12 +178,12 @@ class LearnedCodec:
         nz, ny, nx = shape
         p = min(patch, nz, ny, nx)
         n = nz * ny * nx
-        cw = code_words(shape)
+        cw = code_words(shape, levels)
         code = ginsu.DeviceBuffer(4 * cw)
         deq, half, inv = [], [], []
         for s in snapshots:  # the dequantised twin of every snapshot, its half steps, 1 / its norm
             d, h = ginsu.DeviceBuffer(4 * n), ginsu.DeviceBuffer(4 * n)
-            encode(s, shape, code.ptr)
+            encode(s, shape, code.ptr, levels=levels)
             dequantize(code.ptr, d.ptr)
             if self.lib.wavefd_lcodec_halfstep(code.ptr, h.ptr, None) != 0:
                 raise RuntimeError(_err())
