@@ -39,6 +39,48 @@ def test_device_inject_extract_equals_host(oracle):
         d2 = DeviceArray(data)
         w.extractdata(d2, DeviceArray(field), it, w.receiver_blocking(pts, 4))
         assert np.array_equal(d2.numpy(), d1)
+        d3 = data.copy()  # device field, host data: the call's time sample staged
+        w.extractdata(d3, DeviceArray(field), it, w.receiver_blocking(pts, 4))
+        assert np.array_equal(d3, d1)
+
+
+def test_device_calls_follow_changed_points_and_reject_bad_time_index():
+    # a later call with the same number of points but other coefficients or indices must not
+    # reuse the earlier call's grouped points; a time index outside 1..nt changes nothing
+    import wavefd_amd as w
+    from tests.devmem import DeviceArray
+    from wavefd_amd import _lib
+    rng = np.random.default_rng(4)
+    n, nr, nt = 2000, 6, 9
+    pts = np.zeros(50, w.SourcePoint32)
+    pts["iu"] = rng.integers(1, n + 1, 50)
+    pts["ir"] = rng.integers(1, nr + 1, 50)
+    pts["c"] = rng.standard_normal(50).astype(F32)
+    field = rng.standard_normal(n).astype(F32)
+    data = rng.standard_normal((nr, nt)).astype(F32)
+    for change in ("none", "c", "iu", "ir"):
+        q = pts.copy()
+        if change == "c":
+            q["c"][7] += 1.0
+        elif change == "iu":
+            q["iu"][3] = q["iu"][3] % n + 1
+        elif change == "ir":
+            q["ir"][5] = q["ir"][5] % nr + 1
+        a = field.copy()
+        w.injectdata(a, w.source_blocking(q, 3), data, 4)
+        dev = DeviceArray(field)
+        w.injectdata(dev, w.source_blocking(q, 3), data, 4)
+        assert np.array_equal(dev.numpy(), a), change
+        d1 = data.copy()
+        w.extractdata(d1, field, 4, w.receiver_blocking(q, 3))
+        d2 = data.copy()
+        w.extractdata(d2, DeviceArray(field), 4, w.receiver_blocking(q, 3))
+        assert np.array_equal(d2, d1), change
+    dev = DeviceArray(field)
+    for it in (0, nt + 1):
+        w.injectdata(dev, w.source_blocking(pts, 3), DeviceArray(data), it)
+        assert "time index" in _lib.last_error("spacetime")
+    assert np.array_equal(dev.numpy(), field)
 
 
 def test_device_inject_rejects_points_outside_field():

@@ -7,8 +7,11 @@
 //     accumulation order, because the data is there (one thread: the result does not depend
 //     on the reference's nthreads, which only partitions race-free work);
 //   * device arrays (GetDevicePtr, or any hipMalloc'd buffer) are processed by gfx950
-//     kernels on the null stream, ordered after prior work on that stream and completed
-//     before the call returns.
+//     kernels on the null stream, after all prior work on the device (the propagators'
+//     own streams included) and completed before the call returns.  A call with the same
+//     points as an earlier one reuses that call's grouped points on the device (a host
+//     comparison instead of the sort and the upload), and host data arrays move only the
+//     one time sample the call reads or writes.
 // Injection / extraction accumulate per key in the reference's partition order: all points
 // sharing a grid index (injection) or a trace (extraction) sit in one partition in order
 // (src/spacetime.jl:1087-1129), and the device kernels give one thread per key group.
@@ -17,6 +20,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -51,30 +55,110 @@ bool is_device(const void *ptr) {
     return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
+// data value of trace ir (1-based) at this call's time sample: data[(ir - 1) * stride + col]
+// (a device data array: stride nt, col it - 1; the host's one column staged on the device:
+// stride 1, col 0)
 template <class P, class T>
-__global__ void k_inject(T *__restrict__ field, const T *__restrict__ data, long it, long nt, const P *__restrict__ pts,
-                         const int *__restrict__ gs, int ng) {
+__global__ void k_inject(T *__restrict__ field, const T *__restrict__ data, long stride, long col,
+                         const P *__restrict__ pts, const int *__restrict__ gs, int ng) {
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= ng) return;
     const long iu = pts[gs[gi]].iu - 1;
     T acc = field[iu];
-    for (int i = gs[gi]; i < gs[gi + 1]; i++) acc += pts[i].c * data[(pts[i].ir - 1) * nt + it - 1];
+    for (int i = gs[gi]; i < gs[gi + 1]; i++) acc += pts[i].c * data[(pts[i].ir - 1) * stride + col];
     field[iu] = acc;
 }
 
 template <class P, class T>
-__global__ void k_extract(T *__restrict__ data, const T *__restrict__ field, long it, long nt,
+__global__ void k_extract(T *__restrict__ data, const T *__restrict__ field, long stride, long col,
                           const P *__restrict__ pts, const int *__restrict__ gs, int ng) {
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= ng) return;
-    const long id = (pts[gs[gi]].ir - 1) * nt + it - 1;
+    const long id = (pts[gs[gi]].ir - 1) * stride + col;
     T acc = data[id];
     for (int i = gs[gi]; i < gs[gi + 1]; i++) acc += pts[i].c * field[pts[i].iu - 1];
     data[id] = acc;
 }
 
-// Group the partitions' points by key (stable, so per-key order is the partition order),
-// upload, launch, wait.  Used when the field is device memory.
+// The partitions' points grouped by key (stable, so per-key order is the partition order) on
+// one device: the layout the kernels read.  Kept for later calls with the same points (compared
+// on the host), up to LAYOUT_CACHE per point type; the oldest is freed first.
+template <class P>
+struct Layout {
+    bool inject;
+    int dev;
+    std::vector<P> key;  // the points in partition order, as the caller passed them
+    long maxiu, maxir;
+    int ng;
+    P *dpts;
+    int *dgs;
+    void *dcol;  // maxir data values: a host data array's column, staged (allocated at first use)
+};
+constexpr size_t LAYOUT_CACHE = 8;
+std::mutex g_layout_mu;
+
+template <class P>
+std::vector<Layout<P>> &layouts() {
+    static std::vector<Layout<P>> c;
+    return c;
+}
+
+template <class P>
+void free_layout(Layout<P> &l) {
+    if (l.dpts) (void)hipFree(l.dpts);
+    if (l.dgs) (void)hipFree(l.dgs);
+    if (l.dcol) (void)hipFree(l.dcol);
+    l.dpts = nullptr;
+    l.dgs = nullptr;
+    l.dcol = nullptr;
+}
+
+// the layout of `flat` (caller's order) for this device: cached, or built and uploaded
+template <class P>
+Layout<P> *layout_of(bool inject, int dev, std::vector<P> &flat, hipError_t &e) {
+    e = hipSuccess;
+    auto &c = layouts<P>();
+    // members compared (not the struct's padding bytes); coefficients bit for bit
+    auto same = [](const std::vector<P> &a, const std::vector<P> &b) {
+        if (a.size() != b.size()) return false;
+        for (size_t i = 0; i < a.size(); i++)
+            if (a[i].iu != b[i].iu || a[i].ir != b[i].ir || std::memcmp(&a[i].c, &b[i].c, sizeof(a[i].c)) != 0)
+                return false;
+        return true;
+    };
+    for (auto &l : c)
+        if (l.inject == inject && l.dev == dev && same(l.key, flat)) return &l;
+    Layout<P> l{inject, dev, flat, 0, 0, 0, nullptr, nullptr, nullptr};
+    std::stable_sort(flat.begin(), flat.end(),
+                     [inject](const P &a, const P &b) { return inject ? a.iu < b.iu : a.ir < b.ir; });
+    std::vector<int> gs;
+    for (size_t i = 0; i < flat.size(); i++) {
+        const long key = inject ? flat[i].iu : flat[i].ir;
+        if (i == 0 || key != (inject ? flat[i - 1].iu : flat[i - 1].ir)) gs.push_back((int)i);
+        l.maxiu = std::max(l.maxiu, flat[i].iu);
+        l.maxir = std::max(l.maxir, flat[i].ir);
+    }
+    gs.push_back((int)flat.size());
+    l.ng = (int)gs.size() - 1;
+    e = hipMalloc(&l.dpts, sizeof(P) * flat.size());
+    if (e == hipSuccess) e = hipMalloc(&l.dgs, sizeof(int) * gs.size());
+    if (e == hipSuccess) e = hipMemcpy(l.dpts, flat.data(), sizeof(P) * flat.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(l.dgs, gs.data(), sizeof(int) * gs.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_layout(l);
+        return nullptr;
+    }
+    if (c.size() >= LAYOUT_CACHE) {
+        free_layout(c.front());
+        c.erase(c.begin());
+    }
+    c.push_back(std::move(l));
+    return &c.back();
+}
+
+// Inject or extract with the field in device memory: after all prior work on the device, on the
+// null stream, completed before returning.  Data on the host moves one time sample (the column
+// the call reads, and for extraction writes back); data on the device is read in place.
 template <class P, class T>
 void device_scatter_gather(bool inject, T *field, T *data, size_t it, size_t nt, P **parts, size_t nparts,
                            const long *npts) {
@@ -98,44 +182,57 @@ void device_scatter_gather(bool inject, T *field, T *data, size_t it, size_t nt,
             flat.push_back(q);
         }
     if (flat.empty()) return;
-    std::stable_sort(flat.begin(), flat.end(),
-                     [inject](const P &a, const P &b) { return inject ? a.iu < b.iu : a.ir < b.ir; });
-    std::vector<int> gs;
-    long maxir = 0;
-    for (size_t i = 0; i < flat.size(); i++) {
-        const long key = inject ? flat[i].iu : flat[i].ir;
-        if (i == 0 || key != (inject ? flat[i - 1].iu : flat[i - 1].ir)) gs.push_back((int)i);
-        maxir = std::max(maxir, flat[i].ir);
+    if (it < 1 || it > nt) {
+        report(std::string(inject ? "injectdata" : "extractdata") + ": time index " + std::to_string(it) +
+               " outside 1.." + std::to_string(nt));
+        return;
     }
-    gs.push_back((int)flat.size());
-    P *dpts = nullptr;
-    int *dgs = nullptr;
-    T *ddata = nullptr;
     const bool data_dev = is_device(data);
-    const size_t dbytes = sizeof(T) * (size_t)maxir * nt;
-    hipError_t e = hipMalloc(&dpts, sizeof(P) * flat.size());
-    if (e == hipSuccess) e = hipMalloc(&dgs, sizeof(int) * gs.size());
-    if (e == hipSuccess) e = hipMemcpy(dpts, flat.data(), sizeof(P) * flat.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dgs, gs.data(), sizeof(int) * gs.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !data_dev) {
-        e = hipMalloc(&ddata, dbytes);
-        if (e == hipSuccess) e = hipMemcpy(ddata, data, dbytes, hipMemcpyHostToDevice);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lock(g_layout_mu);
+    Layout<P> *l = e == hipSuccess ? layout_of<P>(inject, dev, flat, e) : nullptr;
+    if (l && data_dev) {  // the device data array must hold every trace the points name
+        hipDeviceptr_t dbase = nullptr;
+        size_t dspan = 0;
+        if (hipMemGetAddressRange(&dbase, &dspan, (hipDeviceptr_t)data) != hipSuccess) {
+            (void)hipGetLastError();
+            report("cannot determine the size of the device data array");
+            return;
+        }
+        const size_t avail = (dspan - (size_t)((char *)data - (char *)dbase)) / sizeof(T);
+        if ((size_t)(l->maxir - 1) * nt + it > avail) {
+            report("trace " + std::to_string(l->maxir) + " outside the device data array");
+            return;
+        }
     }
-    if (e == hipSuccess) {
-        T *dd = data_dev ? data : ddata;
-        const int ng = (int)gs.size() - 1;
+    T *dcol = nullptr;
+    std::vector<T> col;
+    if (l && e == hipSuccess && !data_dev) {  // the host data's one column, staged
+        if (!l->dcol) e = hipMalloc(&l->dcol, sizeof(T) * (size_t)l->maxir);
+        dcol = static_cast<T *>(l->dcol);
+        col.resize((size_t)l->maxir);
+        for (long r = 0; r < l->maxir; r++) col[(size_t)r] = data[(size_t)r * nt + it - 1];
+        if (e == hipSuccess) e = hipMemcpy(dcol, col.data(), sizeof(T) * col.size(), hipMemcpyHostToDevice);
+    }
+    // work already queued on the device (a propagator's stream included) comes first
+    if (l && e == hipSuccess) e = hipDeviceSynchronize();
+    if (l && e == hipSuccess) {
+        T *dd = data_dev ? data : dcol;
+        const long stride = data_dev ? (long)nt : 1, c0 = data_dev ? (long)it - 1 : 0;
         if (inject)
-            k_inject<P, T><<<(ng + 255) / 256, 256>>>(field, dd, (long)it, (long)nt, dpts, dgs, ng);
+            k_inject<P, T><<<(l->ng + 255) / 256, 256>>>(field, dd, stride, c0, l->dpts, l->dgs, l->ng);
         else
-            k_extract<P, T><<<(ng + 255) / 256, 256>>>(dd, field, (long)it, (long)nt, dpts, dgs, ng);
+            k_extract<P, T><<<(l->ng + 255) / 256, 256>>>(dd, field, stride, c0, l->dpts, l->dgs, l->ng);
         e = hipGetLastError();
-        if (e == hipSuccess) e = hipDeviceSynchronize();
-        if (e == hipSuccess && !inject && !data_dev) e = hipMemcpy(data, ddata, dbytes, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+        if (e == hipSuccess && !inject && !data_dev) {
+            e = hipMemcpy(col.data(), dcol, sizeof(T) * col.size(), hipMemcpyDeviceToHost);
+            if (e == hipSuccess)
+                for (long r = 0; r < l->maxir; r++) data[(size_t)r * nt + it - 1] = col[(size_t)r];
+        }
     }
     if (e != hipSuccess) report(std::string(inject ? "injectdata" : "extractdata") + ": " + hipGetErrorString(e));
-    if (dpts) hipFree(dpts);
-    if (dgs) hipFree(dgs);
-    if (ddata) hipFree(ddata);
 }
 
 // ---- injection without allocation (injectdata_float_noalloc) ---------------------------
