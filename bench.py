@@ -816,10 +816,10 @@ def kernel_source_hash():
 def measured_traffic(physics, n):
     """Bytes per launch of the fused step kernel between the L2 and the memory fabric (so
     Infinity-Cache hits included: an upper bound on HBM bytes) from the committed rocprofv3 --pmc
-    runs (profiles/k_fused_step_traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each
+    runs (profiles/k_wide_step_traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each
     scaled by what the counter reads for a plain copy kernel of known size), or None when no
     measurement of this workload on this build of the kernels is committed."""
-    path = os.path.join(ROOT, "profiles", "k_fused_step_traffic.json")
+    path = os.path.join(ROOT, "profiles", "k_wide_step_traffic.json")
     try:
         with open(path) as fh:
             doc = json.load(fh)
