@@ -105,25 +105,27 @@ def test_adjoint_dot_product(kind, fs):
     assert abs(lhs - rhs) <= 1e-5 * max(abs(lhs), abs(rhs)), (lhs, rhs, abs(lhs - rhs) / abs(lhs))
 
 
+# the second shape spans several tiles along z and y and several x-chunks of the default launch
+# (3 z-tiles; 5 y-tiles of the two-wavefield wide tile, 9 of Iso's two-row tile)
+@pytest.mark.parametrize("shape", [(30, 28, 34), (96, 80, 128)])
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
-def test_2000_steps_match_oracle(oracle, kind):
+def test_2000_steps_match_oracle(oracle, kind, shape):
     import wavefd_amd as w
-    shape = (30, 28, 34)
     nx, ny, nz = shape
     nt, dt, d = 2000, 1e-3, 10.0
     model = make_model(kind, shape, 21)
     prop = getattr(w, CLS[kind])(nz=nz, ny=ny, nx=nx, dz=d, dy=d, dx=d, dt=dt, nsponge=8, freesurface=True,
                                  freqQ=10.0, qMin=0.5, qInterior=80.0)
     ocls = {"iso": oracle.OracleIso, "vti": oracle.OracleVTI, "tti": oracle.OracleTTI}[kind]
-    orc = ocls(nx, ny, nz, d, d, d, dt=dt, fs=True, nsponge=8, freqQ=10.0, qMin=0.5, qInterior=80.0, nthread=8)
+    orc = ocls(nx, ny, nz, d, d, d, dt=dt, fs=True, nsponge=8, freqQ=10.0, qMin=0.5, qInterior=80.0, nthread=16)
     for name, arr in model.items():
         host_view(prop, name)[:] = arr.ravel()
         getattr(orc, name)[...] = arr
-    src = w.hickscoeffs(F32(d), F32(d), F32(d), 0.0, 0.0, 0.0, nz, ny, nx, [52.0], [141.5], [163.3])
+    src = w.hickscoeffs(F32(d), F32(d), F32(d), 0.0, 0.0, 0.0, nz, ny, nx, [52.0], [141.5 * ny / 28], [163.3 * nx / 30])
     wavelet = w.causal_ricker(15.0, dt * np.arange(nt))[None, :].astype(F32)
     nr = 24
     rcv = w.linearcoeffs(F32(d), F32(d), F32(d), 0.0, 0.0, 0.0, nz, ny, nx, np.full(nr, 25.0),
-                         np.linspace(50, 220, nr), np.linspace(60, 270, nr))
+                         np.linspace(50, 220 * ny / 28, nr), np.linspace(60, 270 * nx / 30, nr))
     sblocks, rblocks = w.source_blocking(src, 4), w.receiver_blocking(rcv, 4)
     w.coherence(prop, "device")
     w.sync_to_device(prop)

@@ -46,7 +46,7 @@ struct FusedArgs {
     int cxu;       // the kernel's x-loop unrolling (its steps come in multiples of cxu)
     int nzt, nyt;  // tiles along z and y
     int nxc;       // x-chunks
-    int bz, by;    // tiles per block of the XCD-local order (bz = 0: plain z-fastest order)
+    int bz, by;    // tiles per block of the XCD-local order (bz = 0: plain z-fastest, -1: y-fastest)
     int per_xcd;     // workgroups per XCD that run at once (rounds kept in step); 0: no waiting
     unsigned *sync;  // per-XCD count of finished workgroups, zeroed before the launch
     const float *pCur, *mCur;
@@ -475,9 +475,12 @@ __device__ __forceinline__ void tile_coords(const FusedArgs<PHYS> &A, int tile, 
     const int per_xc = A.nzt * A.nyt;
     xc = tile / per_xc;
     const int rem = tile - xc * per_xc;
-    if (A.bz <= 0) {  // z fastest, then y
+    if (A.bz == 0) {  // z fastest, then y
         zt = rem % A.nzt;
         yt = rem / A.nzt;
+    } else if (A.bz < 0) {  // y fastest, then z
+        yt = rem % A.nyt;
+        zt = rem / A.nyt;
     } else {
         // rows of by y-tiles; inside a row, blocks of bz x (rows in the row) tiles, the last
         // row and the last block of a row ragged
@@ -666,13 +669,15 @@ static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float 
     // and TTI plain z-fastest, so that the tiles resident on an XCD hold whole z-columns
     // (measured on MI355X, plain against 4 x 8 blocks: TTI 800^3 10.67 against 11.55 ms, 512^3
     // 3.05 against 3.23 ms, Iso 512^3 1.135 against 1.147 ms; VTI 800^3 8.97 against 8.79 ms).
-    // WAVEFD_FUSED_BZ=k: blocks of k along z; WAVEFD_FUSED_ORDER=0: plain (for measurements)
+    // WAVEFD_FUSED_BZ=k: blocks of k along z; WAVEFD_FUSED_ORDER=0: plain, 2: y-fastest (for measurements)
     const int xcd_cus = std::max(1, p->cu_count / 8);
     int bz = PHYS == VTI ? 4 : 0;
     if (const char *e = std::getenv("WAVEFD_FUSED_BZ")) bz = std::max(0, std::atoi(e));
-    if (const char *e = std::getenv("WAVEFD_FUSED_ORDER"))
+    if (const char *e = std::getenv("WAVEFD_FUSED_ORDER")) {
         if (std::atoi(e) == 0) bz = 0;
-    A.bz = bz > 0 ? std::min(bz, A.nzt) : 0;
+        if (std::atoi(e) == 2) bz = -1;
+    }
+    A.bz = bz > 0 ? std::min(bz, A.nzt) : bz;
     A.by = A.bz > 0 ? std::max(1, xcd_cus / A.bz) : 1;
     const long nxc = (p->nx + A.cx - 1) / A.cx;
     A.nxc = (int)nxc;
