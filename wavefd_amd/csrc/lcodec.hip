@@ -22,18 +22,19 @@
 //     typical blocks, would flatten the sharp peaks of the source region).  1 bit per 512
 //     values.
 // The network's convolutions, forward and backward (data and weights), run on the matrix
-// cores: v_mfma_f32_16x16x16_bf16 with bfloat16 weights and activations and float32
-// accumulation.  Every conv is computed as 16 -> 16 channels with zero-padded weights, so one
-// kernel serves all three layers and their transposes.
+// cores: gfx950's v_mfma_f32_16x16x32_bf16 with bfloat16 weights and activations and float32
+// accumulation (two taps of 16 channels, or 32 voxels of the weight gradient, per instruction;
+// the input tiles staged in LDS).  Every conv is computed as 16 -> 16 channels with zero-padded
+// weights, so one kernel serves all three layers and their transposes.
 //
 // Layouts: a volume batch is npatch volumes of X x Y x Z voxels, z fastest (the propagators'
 // x-slowest layout with X = nx, Y = ny, Z = nz).  Activations and activation gradients are
 // bfloat16, 16 channels per voxel, channel fastest.  Weights W[t][co][c] (t = tap, 27), with
 // the matrix-core B operand layout Wb[t][n][k] (n output, k input channel).
 //
-// Matrix-core operand layout of v_mfma_f32_16x16x16_bf16 (wave64): lane l supplies
-// A[m = l % 16][k = 4 (l / 16) + j] and B[k = 4 (l / 16) + j][n = l % 16] for j = 0..3, and
-// holds D[m = 4 (l / 16) + j][n = l % 16].
+// Matrix-core operand layout of v_mfma_f32_16x16x32_bf16 (wave64): lane l supplies
+// A[m = l % 16][k = 8 (l / 16) + j] and B[k = 8 (l / 16) + j][n = l % 16] for j = 0..7, and
+// holds D[m = 4 (l / 16) + j][n = l % 16], j = 0..3.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -315,13 +316,11 @@ __global__ void __launch_bounds__(256) k_apply_flags(QGeo g, const uint32_t *__r
 }
 
 // ---- convolutions on the matrix cores --------------------------------------------------
-typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 struct Vol {
     long npatch, X, Y, Z;
     __host__ __device__ long nvox() const { return npatch * X * Y * Z; }
-    __host__ __device__ long ntiles() const { return npatch * X * Y * ((Z + 15) / 16); }
 };
 
 // voxel (p, x, y, z) shifted by tap t (dx, dy, dz in -1..1); -1 outside the volume
@@ -332,88 +331,130 @@ __device__ __forceinline__ long shifted(const Vol &v, long p, long x, long y, lo
     return ((p * v.X + xx) * v.Y + yy) * v.Z + zz;
 }
 
-__device__ __forceinline__ s4 ld4(const uint16_t *a) {  // 4 consecutive bfloat16 (8-byte aligned)
-    const uint2 u = *reinterpret_cast<const uint2 *>(a);
-    s4 r;
-    r.x = (short)(u.x & 0xffff);
-    r.y = (short)(u.x >> 16);
-    r.z = (short)(u.y & 0xffff);
-    r.w = (short)(u.y >> 16);
-    return r;
-}
 
 enum ConvOut { OUT_RELU = 0, OUT_RAW = 1, OUT_MASK = 2 };
 
-// out[v][n] = sum_t sum_k in[v + t][k] Wb[t][n][k] (+ epilogue); one wave per 16-voxel z-tile.
+// out[v][n] = sum_t sum_k in[v + t][k] Wb[t][n][k] (+ epilogue), on the matrix cores of gfx950:
+// v_mfma_f32_16x16x32_bf16 takes two taps (2 x 16 input channels) per instruction, 14 per 16-voxel
+// output tile (the 28th tap zero).  A workgroup (4 waves) owns one x-plane, CONV_BY y-rows and 16 z
+// of one volume: it stages the 3 x (CONV_BY + 2) x 18 input voxels around them in LDS (zero outside
+// the volume), each wave keeps the layer's 14 weight operands in registers and computes CONV_BY / 4
+// output tiles, its A operands read from LDS (16 bytes per lane and instruction).
 //   OUT_RELU:     act[v][n] = bf16(relu(acc + bias[n]))
 //   OUT_RAW:      res[v] = acc + bias[0], channel 0 only, float32
 //   OUT_MASK:     act[v][n] = bf16(acc) where mask[v][n] > 0, else 0 (relu backward)
+// Operand layout (wave64): lane l supplies A[m = l % 16][k = 8 (l / 16) + j] and
+// B[k = 8 (l / 16) + j][n = l % 16], j = 0..7, and holds D[m = 4 (l / 16) + j][n = l % 16], j = 0..3;
+// k < 16 is the pair's first tap (channel k), k >= 16 its second (channel k - 16).
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+constexpr int CONV_BY = 8, CONV_SY = CONV_BY + 2, CONV_SZ = 18;
+constexpr int CONV_STAGE = 3 * CONV_SY * CONV_SZ * 2;  // 16-byte halves (8 channels) of the staged voxels
+
+__host__ __device__ inline long conv_blocks(const Vol &v) {
+    return v.npatch * v.X * ((v.Y + CONV_BY - 1) / CONV_BY) * ((v.Z + 15) / 16);
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(256) k_conv(Vol vol, const uint16_t *__restrict__ in, const uint16_t *__restrict__ wb,
                                               const float *__restrict__ bias, const uint16_t *__restrict__ mask,
-                                              const float *__restrict__ base, float norm, uint16_t *__restrict__ act,
-                                              float *__restrict__ res) {
-    const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (tile >= vol.ntiles()) return;
-    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    const long zt = (vol.Z + 15) / 16;
-    const long row = tile / zt, z0 = (tile - row * zt) * 16;
-    const long y = row % vol.Y, x = (row / vol.Y) % vol.X, p = row / (vol.Y * vol.X);
-    const long za = z0 + r;  // this lane's A-row voxel
-    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    const s4 zero = {0, 0, 0, 0};
-    for (int t = 0; t < NT; t++) {
-        const long s = za < vol.Z ? shifted(vol, p, x, y, za, t) : -1;
-        const s4 a = s >= 0 ? ld4(in + s * CH + 4 * g) : zero;
-        const s4 b = ld4(wb + (t * CH + r) * CH + 4 * g);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+                                              uint16_t *__restrict__ act, float *__restrict__ res) {
+    __shared__ uint4 st[CONV_STAGE];
+    const long nzt = (vol.Z + 15) / 16, nyt = (vol.Y + CONV_BY - 1) / CONV_BY;
+    long b = blockIdx.x;
+    const long zt = b % nzt;
+    b /= nzt;
+    const long ytl = b % nyt;
+    b /= nyt;
+    const long x = b % vol.X, p = b / vol.X;
+    const long z0 = zt * 16, y0 = ytl * CONV_BY;
+    for (int i = threadIdx.x; i < CONV_STAGE; i += 256) {
+        const int h = i & 1, v = i >> 1;
+        const int zi = v % CONV_SZ, yi = (v / CONV_SZ) % CONV_SY, xi = v / (CONV_SZ * CONV_SY);
+        const long xx = x + xi - 1, yy = y0 + yi - 1, zz = z0 + zi - 1;
+        uint4 val = make_uint4(0u, 0u, 0u, 0u);
+        if (xx >= 0 && xx < vol.X && yy >= 0 && yy < vol.Y && zz >= 0 && zz < vol.Z)
+            val = *reinterpret_cast<const uint4 *>(in + (((p * vol.X + xx) * vol.Y + yy) * vol.Z + zz) * CH + 8 * h);
+        st[i] = val;
     }
-    const int n = r;  // D column
-    for (int j = 0; j < 4; j++) {
-        const long z = z0 + 4 * g + j;
-        if (z >= vol.Z) continue;
-        const long v = ((p * vol.X + x) * vol.Y + y) * vol.Z + z;
-        const float a = acc[j];
-        if constexpr (MODE == OUT_RELU) {
-            const float h = a + bias[n];
-            act[v * CH + n] = f2bf(h > 0.0f ? h : 0.0f);
-        } else if constexpr (MODE == OUT_RAW) {
-            if (n == 0) res[v] = a + bias[0];
-        } else {
-            act[v * CH + n] = bf2f(mask[v * CH + n]) > 0.0f ? f2bf(a) : (uint16_t)0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    const int hsel = g & 1, tsel = g >> 1;  // channel half, tap of the pair
+    bf8 wreg[14];
+#pragma unroll
+    for (int q = 0; q < 14; q++) {
+        const int t = 2 * q + tsel;
+        const uint4 u = t < NT ? *reinterpret_cast<const uint4 *>(wb + (t * CH + r) * CH + 8 * hsel)
+                               : make_uint4(0u, 0u, 0u, 0u);
+        wreg[q] = __builtin_bit_cast(bf8, u);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < CONV_BY / 4; rr++) {
+        const int yl = w * (CONV_BY / 4) + rr;  // this wave's y-row in the block (uniform)
+        const long y = y0 + yl;
+        if (y >= vol.Y) break;
+        f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int q = 0; q < 14; q++) {
+            const int t = 2 * q + tsel;
+            const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+            const int vi = ((1 + dx) * CONV_SY + (yl + 1 + dy)) * CONV_SZ + (r + 1 + dz);
+            const uint4 u = t < NT ? st[2 * vi + hsel] : make_uint4(0u, 0u, 0u, 0u);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, u), wreg[q], acc, 0, 0, 0);
+        }
+        const int n = r;  // D column
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long z = z0 + 4 * g + j;
+            if (z >= vol.Z) continue;
+            const long v = ((p * vol.X + x) * vol.Y + y) * vol.Z + z;
+            const float a = acc[j];
+            if constexpr (MODE == OUT_RELU) {
+                const float hv = a + bias[n];
+                act[v * CH + n] = f2bf(hv > 0.0f ? hv : 0.0f);
+            } else if constexpr (MODE == OUT_RAW) {
+                if (n == 0) res[v] = a + bias[0];
+            } else {
+                act[v * CH + n] = bf2f(mask[v * CH + n]) > 0.0f ? f2bf(a) : (uint16_t)0;
+            }
         }
     }
 }
 
-// partial weight gradients: G[t][co][c] = sum_v dO[v][co] in[v + t][c] over the tiles of one
-// workgroup (blockIdx.x) for tap blockIdx.y; each wave takes every 4th tile of the range and
-// writes its own partial, part[(blk * 4 + wave) * 27 + t][co][c]
+// partial weight gradients: G[t][co][c] = sum_v dO[v][co] in[v + t][c] over the 32-voxel z-tiles of
+// one workgroup (blockIdx.x) for tap blockIdx.y, v_mfma_f32_16x16x32_bf16 with K = 32 voxels (lane l
+// supplies A[co = l % 16][voxel 8 (l / 16) + j] and B[voxel 8 (l / 16) + j][c = l % 16]); each wave
+// takes every 4th tile of the range and writes its own partial, part[(blk * 4 + wave) * 27 + t][co][c]
+__host__ __device__ inline long wgrad_tiles(const Vol &v) { return v.npatch * v.X * v.Y * ((v.Z + 31) / 32); }
+
 __global__ void __launch_bounds__(256) k_wgrad(Vol vol, const uint16_t *__restrict__ dO, const uint16_t *__restrict__ in,
                                                long tiles_per_blk, float *__restrict__ part) {
     const int t = blockIdx.y, wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    const long zt = (vol.Z + 15) / 16, nt = vol.ntiles();
+    const long zt = (vol.Z + 31) / 32, nt = wgrad_tiles(vol);
     const long t0 = (long)blockIdx.x * tiles_per_blk;
     f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     for (long tile = t0 + wave; tile < t0 + tiles_per_blk && tile < nt; tile += 4) {
-        const long row = tile / zt, z0 = (tile - row * zt) * 16;
+        const long row = tile / zt, z0 = (tile - row * zt) * 32;
         const long y = row % vol.Y, x = (row / vol.Y) % vol.X, p = row / (vol.Y * vol.X);
-        s4 a, b;
-        short av[4], bv[4];
-        for (int j = 0; j < 4; j++) {
-            const long z = z0 + 4 * g + j;  // K index = voxel
+        uint16_t av[8], bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const long z = z0 + 8 * g + j;  // K index = voxel
+            av[j] = 0;
+            bv[j] = 0;
             if (z < vol.Z) {
                 const long v = ((p * vol.X + x) * vol.Y + y) * vol.Z + z;
-                av[j] = (short)dO[v * CH + r];  // A[m = co = r][k = voxel]
-                const long s = shifted(vol, p, x, y, z, t);
-                bv[j] = s >= 0 ? (short)in[s * CH + r] : (short)0;  // B[k = voxel][n = c = r]
-            } else {
-                av[j] = 0;
-                bv[j] = 0;
+                av[j] = dO[v * CH + r];  // A[m = co = r][k = voxel]
+                const long sh = shifted(vol, p, x, y, z, t);
+                if (sh >= 0) bv[j] = in[sh * CH + r];  // B[k = voxel][n = c = r]
             }
         }
-        a.x = av[0]; a.y = av[1]; a.z = av[2]; a.w = av[3];
-        b.x = bv[0]; b.y = bv[1]; b.z = bv[2]; b.w = bv[3];
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        bf8 a, b;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            a[j] = __builtin_bit_cast(__bf16, av[j]);
+            b[j] = __builtin_bit_cast(__bf16, bv[j]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
     }
     float *o = part + ((long)(blockIdx.x * 4 + wave) * NT + t) * CH * CH;
     for (int j = 0; j < 4; j++) o[(4 * g + j) * CH + r] = acc[j];  // D[m = co][n = c]
@@ -606,7 +647,7 @@ long prep(Net *n, hipStream_t s) {
     return 0;
 }
 
-unsigned grid_tiles(const Vol &v) { return (unsigned)((v.ntiles() + 3) / 4); }
+unsigned grid_conv(const Vol &v) { return (unsigned)conv_blocks(v); }
 
 // both halves: the volumes and their negations
 Vol twice(const Vol &v) { return Vol{2 * v.npatch, v.X, v.Y, v.Z}; }
@@ -619,12 +660,11 @@ long forward(Net *n, const Vol &vol, const float *q, float *out, float norm, con
     if (ensure_ws(n, nv)) return -1;
     if (prep(n, s)) return -1;
     k_pack1<<<(unsigned)((nv + 255) / 256), 256, 0, s>>>(nv, q, 1.0f / norm, n->x0);
-    k_conv<OUT_RELU><<<grid_tiles(v2), 256, 0, s>>>(v2, n->x0, n->wb, layer_b(n->p, 0), nullptr, nullptr, 0.0f,
-                                                    n->h1, nullptr);
-    k_conv<OUT_RELU><<<grid_tiles(v2), 256, 0, s>>>(v2, n->h1, n->wb + LAYER_W, layer_b(n->p, 1), nullptr, nullptr,
-                                                    0.0f, n->h2, nullptr);
-    k_conv<OUT_RAW><<<grid_tiles(v2), 256, 0, s>>>(v2, n->h2, n->wb + 2 * LAYER_W, layer_b(n->p, 2), nullptr,
-                                                   nullptr, 0.0f, nullptr, n->raw);
+    k_conv<OUT_RELU><<<grid_conv(v2), 256, 0, s>>>(v2, n->x0, n->wb, layer_b(n->p, 0), nullptr, n->h1, nullptr);
+    k_conv<OUT_RELU><<<grid_conv(v2), 256, 0, s>>>(v2, n->h1, n->wb + LAYER_W, layer_b(n->p, 1), nullptr, n->h2,
+                                                   nullptr);
+    k_conv<OUT_RAW><<<grid_conv(v2), 256, 0, s>>>(v2, n->h2, n->wb + 2 * LAYER_W, layer_b(n->p, 2), nullptr, nullptr,
+                                                  n->raw);
     k_combine<<<(unsigned)((nv + 255) / 256), 256, 0, s>>>(nv, q, n->raw, norm, hs, out);
     LC_TRY(hipGetLastError());
     return 0;
@@ -632,7 +672,7 @@ long forward(Net *n, const Vol &vol, const float *q, float *out, float norm, con
 
 // weight and bias gradients of one layer from its output gradient dO and its input
 long layer_grads(Net *n, const Vol &vol, const uint16_t *dO, const uint16_t *in, int l, hipStream_t s) {
-    const long nt = vol.ntiles(), nv = vol.nvox();
+    const long nt = wgrad_tiles(vol), nv = vol.nvox();
     const long tpb = 256;  // tiles per workgroup
     const long nblk = (nt + tpb - 1) / tpb;
     const long vpb = 4096;
@@ -657,11 +697,9 @@ long loss_grad(Net *n, const Vol &vol, const float *q, const float *target, floa
     // layer 3: dO = d (channel 0), over both halves
     const Vol v2 = twice(vol);
     if (layer_grads(n, v2, n->d, n->h2, 2, s)) return -1;
-    k_conv<OUT_MASK><<<grid_tiles(v2), 256, 0, s>>>(v2, n->d, n->wt + 2 * LAYER_W, nullptr, n->h2, nullptr, 0.0f,
-                                                    n->d2, nullptr);
+    k_conv<OUT_MASK><<<grid_conv(v2), 256, 0, s>>>(v2, n->d, n->wt + 2 * LAYER_W, nullptr, n->h2, n->d2, nullptr);
     if (layer_grads(n, v2, n->d2, n->h1, 1, s)) return -1;
-    k_conv<OUT_MASK><<<grid_tiles(v2), 256, 0, s>>>(v2, n->d2, n->wt + LAYER_W, nullptr, n->h1, nullptr, 0.0f,
-                                                    n->d, nullptr);
+    k_conv<OUT_MASK><<<grid_conv(v2), 256, 0, s>>>(v2, n->d2, n->wt + LAYER_W, nullptr, n->h1, n->d, nullptr);
     if (layer_grads(n, v2, n->d, n->x0, 0, s)) return -1;
     k_apply_mask<<<(NPARAM + 255) / 256, 256, 0, s>>>(NPARAM, n->g, n->mask);  // no padding, no biases
     LC_TRY(hipGetLastError());
