@@ -91,7 +91,8 @@ struct SourcePoint32; /* spacetime.h */
     /* step n (1..8) distinct handles of this physics, all in DEVICE mode, by one launch: the     \
      * same results as calling on each handle i TimeStep (epilogue 0), TimeStepSaveDP (1: a0[i] =  \
      * DP, a1[i] = DM), TimeStepBornInjection_V (2: a0 = dV, a1 = DP, a2 = DM) or                  \
-     * TimeStepAdjointBornAccumulation_V (3: a0 = g, a1 = DP, a2 = DM); arrays device memory,     \
+     * TimeStepAdjointBornAccumulation_V (3: a0 = g, a1 = DP, a2 = DM) or, Iso only,             \
+     * TimeStepBornCoupled_V (4: a0 = dV, a1 = DP); arrays device memory,                         \
      * DM lists only for VTI/TTI.  Runs after the work queued on every handle's stream; later     \
      * work on each handle's stream runs after it.  0 on success. */                              \
     long P##_TimeStepBatch(void **ps, long n, long epilogue, float **a0, float **a1, float **a2);  \

@@ -1,8 +1,9 @@
 """GPU: several grids stepped by one launch (TimeStepBatch) equal the same grids stepped one by
-one, bit for bit, for every epilogue (plain step, save DP, Born injection, imaging), for Iso,
-VTI and TTI, both tile shapes, grids of different sizes with and without a free surface; the
-batch's argument checks; and the batched Born driver (born_shots) gives the same Born data and
-image as the shots run one after another (born_shot)."""
+one, bit for bit, for every epilogue (plain step, save DP, Born injection, imaging, and Iso's
+coupled background + Born step), for Iso, VTI and TTI, every tile shape (the wide tiles through
+the wide batch kernel), grids of different sizes with and without a free surface; the batch's
+argument checks; and the batched Born driver (born_shots) gives the same Born data and image as
+the shots run one after another (born_shot)."""
 import ctypes
 
 import numpy as np
@@ -43,12 +44,24 @@ def _fields(kind):
     return ("pCur", "pOld") + (("mCur", "mOld") if kind != "iso" else ())
 
 
+def _dev_field(p, name, n):
+    import wavefd_amd as w
+    from wavefd_amd import ginsu
+    out = np.zeros(n, F32)
+    w.synchronize(p)
+    ginsu.copy(out.ctypes.data, w.device_ptr(p, name), 4 * n)
+    ginsu.sync()
+    return out
+
+
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
-@pytest.mark.parametrize("tile", ["64x16", "32x32"])
-@pytest.mark.parametrize("epi", [0, 1, 2, 3])
+@pytest.mark.parametrize("tile", ["64x16", "32x32", "wide", "wide2", "auto"])
+@pytest.mark.parametrize("epi", [0, 1, 2, 3, 4])
 def test_batch_equals_one_by_one(kind, tile, epi):
     import wavefd_amd as w
     from wavefd_amd import _lib
+    if epi == 4 and kind != "iso":
+        pytest.skip("the coupled Born step is Iso's")
     nf = 1 if kind == "iso" else 2
     a = [_prop(kind, s, fs, 3 + i, tile) for i, (s, fs) in enumerate(GRIDS)]
     b = [_prop(kind, s, fs, 3 + i, tile) for i, (s, fs) in enumerate(GRIDS)]
@@ -61,6 +74,8 @@ def test_batch_equals_one_by_one(kind, tile, epi):
             host = []
         elif epi == 1:
             host = [np.zeros(n, F32) for _ in range(nf)]
+        elif epi == 4:  # dV, DP out
+            host = [smooth(rng, s, 40.0, 2).ravel(), np.zeros(n, F32)]
         else:
             host = [smooth(rng, s, 40.0 if epi == 2 else 1.0, 2).ravel()] + [smooth(rng, s, 1.0, 2).ravel()
                                                                             for _ in range(nf)]
@@ -78,8 +93,10 @@ def test_batch_equals_one_by_one(kind, tile, epi):
             assert lib.TimeStepSaveDP(p.p, *ptrs) == 0
         elif epi == 2:
             assert lib.TimeStepBornInjection_V(p.p, *ptrs) == 0
-        else:
+        elif epi == 3:
             assert lib.TimeStepAdjointBornAccumulation_V(p.p, *ptrs) == 0
+        else:
+            assert lib.TimeStepBornCoupled_V(p.p, *ptrs) == 0
     # all at once
     n = len(b)
     hs = (ctypes.c_void_p * n)(*[p.p for p in b])
@@ -92,6 +109,10 @@ def test_batch_equals_one_by_one(kind, tile, epi):
     for i, (s, _) in enumerate(GRIDS):
         for nm in _fields(kind):
             assert np.array_equal(host_view(b[i], nm), host_view(a[i], nm)), (i, nm)
+        if epi == 4:  # the Born field
+            for nm in ("mCur", "mOld"):
+                n = int(np.prod(s))
+                assert np.array_equal(_dev_field(b[i], nm, n), _dev_field(a[i], nm, n)), (i, nm)
         if epi == 0:
             assert np.array_equal(host_view(b[i], "pSpace"), host_view(a[i], "pSpace"))
         for k, (da, db) in enumerate(zip(arrs_a[i], arrs_b[i])):
@@ -117,6 +138,12 @@ def test_batch_argument_checks():
     assert call(a, 0) == -1 and "between 1 and 8" in _lib.last_error("iso")
     assert call([a[0], a[0]], 2) == -1 and "twice" in _lib.last_error("iso")
     assert call(a, 2, 5) == -1 and "epilogue" in _lib.last_error("iso")
+    v = [_prop("vti", s, fs, 3, "auto") for s, fs in GRIDS[:2]]
+    hv = (ctypes.c_void_p * 2)(*[p.p for p in v])
+    assert v[0]._lib.TimeStepBatch(hv, 2, 4, None, None, None) == -1 and "epilogue" in _lib.last_error("vti")
+    v[0]._lib.ClearError()
+    for p in v:
+        w.free(p)
     host = np.zeros(int(np.prod(GRIDS[0][0])), F32)
     col = (ctypes.c_void_p * 2)(host.ctypes.data, host.ctypes.data)
     assert call(a, 2, 1, (col, None, None)) == -1 and "device memory" in _lib.last_error("iso")

@@ -69,16 +69,15 @@ def test_checkpointed_born_equals_exact_storage(kind, form):
 @pytest.mark.parametrize("kind", ["iso", "vti"])
 def test_predictive_batch_equals_predictive_sequential(kind):
     # the predictive store is lossy, but deterministic: a batch of shots gives each shot the
-    # Born data and image it gets alone with the same passes (separate background and Born
-    # passes; alone, Iso would couple them and form its Born source from the exact DP instead
-    # of the store's reconstruction)
+    # Born data and image it gets alone with the same passes (Iso: the coupled background + Born
+    # pass, whose Born source is formed from the exact DP inside the step, in the batch as alone)
     from wavefd_amd import born, ginsu
     nt = 40
     shape, model, dvb, windows, shots, keep = _setup(kind, nt, nshots=3)
     img_s = ginsu.DeviceBuffer(4 * int(np.prod(shape)))
     img_s.zero()
-    seq = [born.born_shot(kind, w, SP, 1e-3, s, model, dvb.ptr, img_s.ptr, nsponge=8, predictive=(8, 3, 16),
-                          coupled=False)["born_data"] for w, s in zip(windows, shots)]
+    seq = [born.born_shot(kind, w, SP, 1e-3, s, model, dvb.ptr, img_s.ptr, nsponge=8, predictive=(8, 3, 16))
+           ["born_data"] for w, s in zip(windows, shots)]
     img_b = ginsu.DeviceBuffer(4 * int(np.prod(shape)))
     img_b.zero()
     bat = born.born_shots(kind, windows, SP, 1e-3, shots, model, dvb.ptr, img_b.ptr, nsponge=8, batch=3,

@@ -70,6 +70,13 @@ class _DevArray:
         return self.ptr
 
 
+class _Slice:
+    """a piece of a device buffer: the address and size an exact store needs (not owned)"""
+
+    def __init__(self, ptr, nbytes):
+        self.ptr, self.nbytes = ptr, nbytes
+
+
 class _ExactStore:
     """DP_n (DM_n) kept exactly: one float32 field per step and wavefield"""
 
@@ -767,12 +774,22 @@ def _batch_step(runs, epilogue, lists):
 def _run_coupled(runs):
     """passes 1 and 2 of Iso shots as one: TimeStepBornCoupled_V steps the background (saving
     DP_n) and the Born field (with the source of that DP_n) together; the shot's source goes into
-    the background, the receivers read the Born field"""
+    the background, the receivers read the Born field.  Several shots: each step of all of them
+    one TimeStepBatch launch (epilogue 4)"""
     nt = runs[0].nt
     for r in runs:
         r.setup_forward()
         prop3d.set_receivers(r.prop, [r.rcv_pts], nt, fields=2)
     for it in range(1, nt + 1):
+        if len(runs) > 1:
+            for r in runs:
+                r.begin(it)
+            _batch_step(runs, 4, [[r.dv.ptr for r in runs], [r.snap(0, it) for r in runs]])
+            for r in runs:
+                r.commit(it)
+                r.lib.InjectRegistered(r.prop.p, it)
+                r.lib.ExtractRegistered(r.prop.p, it)
+            continue
         for r in runs:
             lib, p = r.lib, r.prop.p
             r.begin(it)
@@ -786,12 +803,12 @@ def _run_coupled(runs):
 
 def _run_passes(runs, fused, adjoint_data, keep_born_data, coupled=False):
     """the three passes over a batch of shots of equal nt, in lockstep; with fused and several
-    shots, each step of all shots is one launch (TimeStepBatch); coupled (Iso, fused, one shot at
-    a time): passes 1 and 2 as one pass over both wavefields"""
+    shots, each step of all shots is one launch (TimeStepBatch); coupled (Iso, fused): passes 1
+    and 2 as one pass over both wavefields"""
     nt = runs[0].nt
     batched = fused and len(runs) > 1
-    if coupled and (batched or not fused or runs[0].nf != 1):
-        raise ValueError("the coupled Born pass is for Iso shots run one at a time with fused steps")
+    if coupled and (not fused or runs[0].nf != 1):
+        raise ValueError("the coupled Born pass is for Iso shots with fused steps")
     if coupled:
         _run_coupled(runs)
     # with checkpointed stores, passes 1 and 2 alternate per segment: the background of a segment
@@ -956,22 +973,44 @@ def born_shots(physics, windows, spacing, dt, shots, model, dmodel, image, nspon
     if cur:
         groups.append(cur)
     data = [None] * len(shots)
+    # exact storage: one buffer for the largest batch's snapshots, sliced per batch (no allocation
+    # per shot)
+    need = [(4 * w.ncells * nt * nf + 255) // 256 * 256 for w in windows]
+    pool = (ginsu.DeviceBuffer(max(sum(need[i] for i in g) for g in groups))
+            if groups and checkpoint is None and predictive is None else None)
+    try:
+        _run_groups(physics, windows, spacing, dt, shots, model, dmodel, image, nsponge, freesurface, freqQ, qMin,
+                    qInterior, adjoint_data, keep_born_data, fused, checkpoint, predictive, groups, need, pool, data)
+    finally:
+        if pool is not None:
+            pool.free()
+    return data
+
+
+def _run_groups(physics, windows, spacing, dt, shots, model, dmodel, image, nsponge, freesurface, freqQ, qMin,
+                qInterior, adjoint_data, keep_born_data, fused, checkpoint, predictive, groups, need, pool, data):
+    """born_shots' batches, one after the other"""
     for g in groups:
         runs = []
         try:
+            off = 0
             for i in g:
+                snap = None
+                if pool is not None:
+                    snap = _Slice(pool.ptr + off, need[i])
+                    off += need[i]
                 runs.append(_ShotRun(physics, windows[i], spacing, dt, shots[i], model, dmodel, nsponge, freesurface,
-                                     freqQ, qMin, qInterior, None, checkpoint=checkpoint, fused=fused,
+                                     freqQ, qMin, qInterior, snap, checkpoint=checkpoint, fused=fused,
                                      predictive=predictive))
             _run_passes(runs, fused, None if adjoint_data is None else [adjoint_data[i] for i in g], keep_born_data,
-                        coupled=fused and physics == "iso" and len(runs) == 1)
+                        coupled=fused and physics == "iso" and
+                        (len(runs) == 1 or not any(isinstance(r.store, _CheckpointStore) for r in runs)))
             for i, r in zip(g, runs):
                 r.finish(image)
                 data[i] = r.born_data
         finally:
             for r in runs:
                 r.free()
-    return data
 
 
 def model_shot(physics, window, spacing, dt, shot, model, nsponge=20, freesurface=True, freqQ=5.0, qMin=0.1,

@@ -884,11 +884,13 @@ static inline bool launch_fused_batch(Prop **ps, int n, float *const *e0, float 
 
 template <int PHYS, int EPI>
 static inline bool launch_fused_batch_tile(Prop **ps, int n, float *const *e0, float *const *e1, float *const *e2) {
-    // the several-grid launch has the one-thread-per-cell tiles only: the wide tile runs as 64 x 16,
-    // the two-row wide tile of the plain Iso step as 32 x 32 (that step's tile before it)
+    // the first handle's tile for all: the wide tiles (the automatic choice) through k_wide_batch,
+    // the one-thread-per-cell tiles through k_fused_batch
     const int tile = fused_tile_of(ps[0], EPI);
-    if (tile == TILE_32x32 || (EPI == EPI_NONE && tile == TILE_64x16W))
-        return launch_fused_batch<PHYS, 32, 32, EPI>(ps, n, e0, e1, e2);
+    if constexpr (fused_nf(PHYS, EPI) == 1)
+        if (tile == TILE_64x16W) return launch_wide_batch<PHYS, EPI, 2>(ps, n, e0, e1, e2);
+    if (tile == TILE_64x32W) return launch_wide_batch<PHYS, EPI>(ps, n, e0, e1, e2);
+    if (tile == TILE_32x32) return launch_fused_batch<PHYS, 32, 32, EPI>(ps, n, e0, e1, e2);
     return launch_fused_batch<PHYS, 64, 16, EPI>(ps, n, e0, e1, e2);
 }
 
@@ -897,7 +899,13 @@ static inline bool step_fused_batch(Prop **ps, int n, float *const *e0, float *c
     for (int i = 0; i < n; i++)
         if (!fused_fits(ps[i])) return false;
     bool ok;
-    if (ps[0]->phys == ISO)
+    if constexpr (EPI == EPI_BORN_COUPLED) {
+        if (ps[0]->phys != ISO) {
+            set_error("the coupled Born step is defined for Iso only");
+            return false;
+        }
+        ok = launch_fused_batch_tile<ISO, EPI>(ps, n, e0, e1, e2);
+    } else if (ps[0]->phys == ISO)
         ok = launch_fused_batch_tile<ISO, EPI>(ps, n, e0, e1, e2);
     else if (ps[0]->phys == VTI)
         ok = launch_fused_batch_tile<VTI, EPI>(ps, n, e0, e1, e2);

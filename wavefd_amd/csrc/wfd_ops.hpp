@@ -507,11 +507,9 @@ static inline bool born_step(Prop *p, int epi, float *a0, float *a1, float *a2) 
 // on a handle with the same model whose field is the Born field (mCur/mOld, allocated and
 // zeroed at the first call; registered sources with fields bit 2 and receivers with fields 2
 // then address it).  One pass reads the model once for both fields and forms DP in registers.
-static inline bool born_coupled(Prop *p, float *dV, float *DP) {
-    if (p->phys != ISO) {
-        set_error("the coupled Born step is defined for Iso only");
-        return false;
-    }
+// Iso: the coupled Born step's second wavefield (the Born field) in the mCur / mOld slots,
+// allocated and zeroed at its first use
+static inline bool ensure_born_field(Prop *p) {
     if (!p->born_field) {
         for (int i : {F_MCUR, F_MOLD}) {
             WFD_TRY(hipMalloc(&p->f[i].d, sizeof(float) * p->n));
@@ -520,6 +518,15 @@ static inline bool born_coupled(Prop *p, float *dV, float *DP) {
         }
         p->born_field = true;
     }
+    return true;
+}
+
+static inline bool born_coupled(Prop *p, float *dV, float *DP) {
+    if (p->phys != ISO) {
+        set_error("the coupled Born step is defined for Iso only");
+        return false;
+    }
+    if (!ensure_born_field(p)) return false;
     if (!p->begin(p->all_mask())) return false;
     Prop::Arg a[2];
     WFD_ARG(dV, 0, true, false, a[0]);
@@ -565,8 +572,9 @@ static inline bool time_step_batch(Physics phys, void **hs, long n, long epi, fl
         set_error("TimeStepBatch: between 1 and " + std::to_string(FUSED_MAX_BATCH) + " handles");
         return false;
     }
-    if (epi < EPI_NONE || epi > EPI_IMAGE) {
-        set_error("TimeStepBatch: epilogue must be 0 (none), 1 (save DP), 2 (Born injection V) or 3 (imaging V)");
+    if (epi < EPI_NONE || epi > EPI_BORN_COUPLED || (epi == EPI_BORN_COUPLED && phys != ISO)) {
+        set_error("TimeStepBatch: epilogue must be 0 (none), 1 (save DP), 2 (Born injection V), 3 (imaging V) or, "
+                  "for Iso, 4 (the coupled Born step)");
         return false;
     }
     Prop *ps[FUSED_MAX_BATCH];
@@ -589,8 +597,9 @@ static inline bool time_step_batch(Physics phys, void **hs, long n, long epi, fl
         all_fused = all_fused && ps[i]->fused;
     }
     const bool two = phys != ISO;
-    // arrays the epilogue needs: SAVE_DP (DP[, DM]) = a0[, a1]; others (x, DP[, DM]) = a0, a1[, a2]
-    const int need = epi == EPI_NONE ? 0 : (epi == EPI_SAVE_DP ? 1 : 2) + (two ? 1 : 0);
+    // arrays the epilogue needs: SAVE_DP (DP[, DM]) = a0[, a1]; coupled (dV, DP) = a0, a1; others
+    // (x, DP[, DM]) = a0, a1[, a2]
+    const int need = epi == EPI_NONE ? 0 : epi == EPI_BORN_COUPLED ? 2 : (epi == EPI_SAVE_DP ? 1 : 2) + (two ? 1 : 0);
     float **arr[3] = {a0, a1, a2};
     for (int k = 0; k < need; k++) {
         if (!arr[k]) {
@@ -606,12 +615,16 @@ static inline bool time_step_batch(Physics phys, void **hs, long n, long epi, fl
     auto get = [&](int k, long i) -> float * { return k < need ? arr[k][i] : nullptr; };
     if (!all_fused) {  // the same results, one handle after the other
         for (long i = 0; i < n; i++) {
-            const bool ok = epi == EPI_NONE ? time_step(ps[i], false)
-                                            : born_step(ps[i], (int)epi, get(0, i), get(1, i), get(2, i));
+            const bool ok = epi == EPI_NONE           ? time_step(ps[i], false)
+                            : epi == EPI_BORN_COUPLED ? born_coupled(ps[i], get(0, i), get(1, i))
+                                                      : born_step(ps[i], (int)epi, get(0, i), get(1, i), get(2, i));
             if (!ok) return false;
         }
         return true;
     }
+    if (epi == EPI_BORN_COUPLED)
+        for (long i = 0; i < n; i++)
+            if (!ensure_born_field(ps[i])) return false;
     hipStream_t s0 = ps[0]->stream;
     for (long i = 0; i < n; i++)
         if (!ps[i]->ev) WFD_TRY(hipEventCreateWithFlags(&ps[i]->ev, hipEventDisableTiming));
@@ -624,15 +637,19 @@ static inline bool time_step_batch(Physics phys, void **hs, long n, long epi, fl
     for (int k = 0; k < need; k++)
         for (long i = 0; i < n; i++) e[k][i] = arr[k][i];
     const int nn = (int)n;
-    const bool ok = epi == EPI_NONE      ? step_fused_batch<EPI_NONE>(ps, nn, nullptr, nullptr, nullptr)
-                    : epi == EPI_SAVE_DP ? step_fused_batch<EPI_SAVE_DP>(ps, nn, e[0], e[1], nullptr)
-                    : epi == EPI_BORN_INJ ? step_fused_batch<EPI_BORN_INJ>(ps, nn, e[0], e[1], e[2])
-                                          : step_fused_batch<EPI_IMAGE>(ps, nn, e[0], e[1], e[2]);
+    const bool ok = epi == EPI_NONE        ? step_fused_batch<EPI_NONE>(ps, nn, nullptr, nullptr, nullptr)
+                    : epi == EPI_SAVE_DP   ? step_fused_batch<EPI_SAVE_DP>(ps, nn, e[0], e[1], nullptr)
+                    : epi == EPI_BORN_INJ  ? step_fused_batch<EPI_BORN_INJ>(ps, nn, e[0], e[1], e[2])
+                    : epi == EPI_IMAGE     ? step_fused_batch<EPI_IMAGE>(ps, nn, e[0], e[1], e[2])
+                                           : step_fused_batch<EPI_BORN_COUPLED>(ps, nn, e[0], e[1], nullptr);
     if (!ok) return false;
     WFD_TRY(hipEventRecord(ps[0]->ev, s0));
     for (long i = 1; i < n; i++)
         if (ps[i]->stream != s0) WFD_TRY(hipStreamWaitEvent(ps[i]->stream, ps[0]->ev, 0));
-    for (long i = 0; i < n; i++) ps[i]->swap_p();
+    for (long i = 0; i < n; i++) {
+        ps[i]->swap_p();
+        if (epi == EPI_BORN_COUPLED) std::swap(ps[i]->f[F_MCUR], ps[i]->f[F_MOLD]);  // the Born field's pair
+    }
     return true;
 }
 

@@ -584,4 +584,45 @@ static inline bool launch_wide(Prop *p, bool linear, float *e0, float *e1, float
     return true;
 }
 
+// Several grids stepped by one launch of the wide kernel: grid s owns workgroups [wg[s], wg[s+1]),
+// each range a multiple of 8 long (the XCD-aware numbering holds inside every range), with its
+// own sizes, arrays and chunking (each grid's own rule); no rounds in step.  The grids share the
+// physics, the rows per thread and the epilogue.
+template <int PHYS, int EPI, int R>
+__global__ void __launch_bounds__(WIDE_THREADS, wide_wpe(fused_nf(PHYS, EPI), R)) k_wide_batch(FusedBatch<PHYS> B) {
+    __shared__ WideLds<fused_nf(PHYS, EPI), R> S;
+    const int bid = (int)blockIdx.x;
+    int s = 0;
+    while (s + 1 < B.n && bid >= B.wg[s + 1]) s++;
+    const FusedArgs<PHYS> &A = B.a[s];
+    int loc, zt, yt, xc;
+    if (!fused_tile_of_wg(A, bid - B.wg[s], loc, zt, yt, xc)) return;
+    if (A.g.fs && zt == 0)
+        wide_body<PHYS, R, EPI, true>(A, S, zt, yt, xc);
+    else
+        wide_body<PHYS, R, EPI, false>(A, S, zt, yt, xc);
+}
+
+template <int PHYS, int EPI, int R = wide_r(PHYS, EPI)>
+static inline bool launch_wide_batch(Prop **ps, int n, float *const *e0, float *const *e1, float *const *e2) {
+    FusedBatch<PHYS> B = {};
+    B.n = n;
+    constexpr bool two_per_cu = wide_wpe(fused_nf(PHYS, EPI), R) == 4;
+    long wg = 0;
+    for (int i = 0; i < n; i++) {
+        B.wg[i] = (int)wg;
+        const long t = fill_fused<PHYS, 64, 8 * R, EPI>(ps[i], false, e0 ? e0[i] : nullptr, e1 ? e1[i] : nullptr,
+                                                       e2 ? e2[i] : nullptr, B.a[i], 8,
+                                                       two_per_cu ? wide_cx2 : wide_cx);
+        wg += (t + 7) / 8 * 8;
+        if (wg >= (1L << 31)) {
+            set_error("batched step: too many workgroups");
+            return false;
+        }
+    }
+    B.wg[n] = (int)wg;
+    k_wide_batch<PHYS, EPI, R><<<dim3((unsigned)wg), dim3(WIDE_THREADS), 0, ps[0]->stream>>>(B);
+    return true;
+}
+
 }  // namespace wfd
