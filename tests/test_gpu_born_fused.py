@@ -4,7 +4,7 @@ TimeStepSaveDP          = TimeStep; ScaleSpatialDerivatives      (DP, DM vs the 
 TimeStepBornInjection_V = TimeStep; ForwardBornInjection_V       (new pCur, mCur)
 TimeStepAdjointBornAccumulation_V = TimeStep; AdjointBornAccumulation_V (g, new pCur, mCur)
 
-for Iso, VTI and TTI, with and without a free surface, on a grid whose tiles are ragged, with
+for Iso, VTI and TTI, with and without a free surface, on grids whose tiles are ragged (and whose last tiles write the annulus from their halo), with
 the fused kernel's tile shapes (64x16, 32x32, wide, wide2) and with the two-pass fallback
 (SetFusedStep 0); and the Born driver gives the same Born data and image with the fused pairs
 as with the separate calls.
@@ -19,16 +19,24 @@ pytestmark = pytest.mark.gpu
 F32 = np.float32
 CLS = {"iso": "Prop3DAcoIsoDenQ_DEO2_FDTD", "vti": "Prop3DAcoVTIDenQ_DEO2_FDTD", "tti": "Prop3DAcoTTIDenQ_DEO2_FDTD"}
 SHAPE = (37, 41, 45)
+# the last two: the last tile along z and y writes the grid's last (annulus) cells from its upper
+# halo (a few cells past a multiple of the outputs per tile for every tile shape)
+SHAPES = [SHAPE, (21, 37, 60), (23, 28, 53)]
 
 
-def _prop(kind, fs, fused):
+@pytest.fixture(params=SHAPES, ids=["x".join(map(str, s)) for s in SHAPES])
+def shape(request):
+    return request.param
+
+
+def _prop(kind, fs, fused, shape=SHAPE):
     import wavefd_amd as w
-    nx, ny, nz = SHAPE
+    nx, ny, nz = shape
     p = getattr(w, CLS[kind])(nz=nz, ny=ny, nx=nx, dz=10.0, dy=11.0, dx=9.0, dt=1e-3, nsponge=6, freesurface=fs,
                               freqQ=10.0, qMin=0.5, qInterior=80.0)
-    for name, arr in make_model(kind, SHAPE, 3).items():
+    for name, arr in make_model(kind, shape, 3).items():
         host_view(p, name)[:] = arr.ravel()
-    for name, arr in make_fields(kind, SHAPE, 5).items():
+    for name, arr in make_fields(kind, shape, 5).items():
         host_view(p, name)[:] = arr.ravel()
     p._lib.SetFusedStep(p.p, 1 if fused else 0)
     if fused:
@@ -36,20 +44,20 @@ def _prop(kind, fs, fused):
     return p
 
 
-def _arrs(kind, seed):
+def _arrs(kind, seed, shape=SHAPE):
     rng = np.random.default_rng(seed)
     n = 1 if kind == "iso" else 2
-    return [smooth(rng, SHAPE, 1.0, 2).ravel().copy() for _ in range(n)]
+    return [smooth(rng, shape, 1.0, 2).ravel().copy() for _ in range(n)]
 
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [True, False])
 @pytest.mark.parametrize("fused", [1, 2, 3, 4, 0])  # tile 64x16, 32x32, wide, wide2, two-pass
-def test_save_dp(kind, fs, fused):
-    a, b = _prop(kind, fs, fused), _prop(kind, fs, fused)
+def test_save_dp(kind, fs, fused, shape):
+    a, b = _prop(kind, fs, fused, shape), _prop(kind, fs, fused, shape)
     a._lib.TimeStep(a.p)
     a._lib.ScaleSpatialDerivatives(a.p)
-    out = [np.zeros(np.prod(SHAPE), F32) for _ in range(1 if kind == "iso" else 2)]
+    out = [np.zeros(np.prod(shape), F32) for _ in range(1 if kind == "iso" else 2)]
     assert b._lib.TimeStepSaveDP(b.p, *[o.ctypes.data for o in out]) == 0
     assert np.array_equal(out[0], host_view(a, "pSpace"))
     if kind != "iso":
@@ -61,11 +69,11 @@ def test_save_dp(kind, fs, fused):
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [True, False])
 @pytest.mark.parametrize("fused", [1, 2, 3, 4, 0])  # tile 64x16, 32x32, wide, wide2, two-pass
-def test_born_injection(kind, fs, fused):
-    a, b = _prop(kind, fs, fused), _prop(kind, fs, fused)
+def test_born_injection(kind, fs, fused, shape):
+    a, b = _prop(kind, fs, fused, shape), _prop(kind, fs, fused, shape)
     rng = np.random.default_rng(8)
-    dv = smooth(rng, SHAPE, 50.0, 2).ravel().copy()
-    dps = _arrs(kind, 9)
+    dv = smooth(rng, shape, 50.0, 2).ravel().copy()
+    dps = _arrs(kind, 9, shape)
     a._lib.TimeStep(a.p)
     a._lib.ForwardBornInjection_V(a.p, dv.ctypes.data, *[d.ctypes.data for d in dps])
     assert b._lib.TimeStepBornInjection_V(b.p, dv.ctypes.data, *[d.ctypes.data for d in dps]) == 0
@@ -76,11 +84,11 @@ def test_born_injection(kind, fs, fused):
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [True, False])
 @pytest.mark.parametrize("fused", [1, 2, 3, 4, 0])  # tile 64x16, 32x32, wide, wide2, two-pass
-def test_adjoint_accumulation(kind, fs, fused):
-    a, b = _prop(kind, fs, fused), _prop(kind, fs, fused)
+def test_adjoint_accumulation(kind, fs, fused, shape):
+    a, b = _prop(kind, fs, fused, shape), _prop(kind, fs, fused, shape)
     rng = np.random.default_rng(10)
-    g0 = smooth(rng, SHAPE, 1.0, 2).ravel().copy()
-    dps = _arrs(kind, 11)
+    g0 = smooth(rng, shape, 1.0, 2).ravel().copy()
+    dps = _arrs(kind, 11, shape)
     ga, gb = g0.copy(), g0.copy()
     a._lib.TimeStep(a.p)
     a._lib.AdjointBornAccumulation_V(a.p, ga.ctypes.data, *[d.ctypes.data for d in dps])
@@ -93,16 +101,16 @@ def test_adjoint_accumulation(kind, fs, fused):
 
 @pytest.mark.parametrize("fs", [True, False])
 @pytest.mark.parametrize("fused", [1, 2, 3, 4, 0])  # tile 64x16, 32x32, wide, wide2, two-pass
-def test_coupled_step(fs, fused):
+def test_coupled_step(fs, fused, shape):
     """TimeStepBornCoupled_V on one handle = TimeStepSaveDP on a background handle followed by
     TimeStepBornInjection_V on a handle holding the Born field, step after step"""
     import wavefd_amd as w
-    a, b, c = _prop("iso", fs, fused), _prop("iso", fs, fused), _prop("iso", fs, fused)
-    n = int(np.prod(SHAPE))
+    a, b, c = _prop("iso", fs, fused, shape), _prop("iso", fs, fused, shape), _prop("iso", fs, fused, shape)
+    n = int(np.prod(shape))
     for nm in ("pCur", "pOld"):  # the Born field starts at zero, like the coupled handle's
         host_view(b, nm)[:] = 0.0
     rng = np.random.default_rng(12)
-    dv = smooth(rng, SHAPE, 50.0, 2).ravel().copy()
+    dv = smooth(rng, shape, 50.0, 2).ravel().copy()
     for step in range(3):
         dpa, dpc = np.zeros(n, F32), np.zeros(n, F32)
         assert a._lib.TimeStepSaveDP(a.p, dpa.ctypes.data) == 0
