@@ -90,10 +90,10 @@ def test_wide_tile_short_chunks_bit_exact(oracle, kind, chunk, tile):
 
 
 @pytest.mark.parametrize("fs", [False, True])
-@pytest.mark.parametrize("tile", ["auto", "wide"])
-def test_vti_linear_step_bit_exact(oracle, fs, tile):
+@pytest.mark.parametrize("tile", ["auto", "wide", "64x16", "32x32"])
+@pytest.mark.parametrize("shape", [(30, 29, 33), (21, 37, 60)])
+def test_vti_linear_step_bit_exact(oracle, fs, tile, shape):
     import wavefd_amd as w
-    shape = (30, 29, 33)
     prop, orc = _pair(oracle, "vti", shape, fs, 3)
     w.set_fused_tile(prop, tile)
     for _ in range(4):
