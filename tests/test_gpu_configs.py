@@ -1,5 +1,7 @@
 """GPU: the HIP path at the configs' own workloads (parity by size-independent properties).
 
+* Iso 256^3 at 10 m, 2000 steps with a Hicks point source and 256 x 256 receivers at 20 m depth:
+  traces and wavefields bit-identical to the oracle running the same loop on the host;
 * VTI 512^3 at 12.5 m, 3000 steps of the linear step (Prop3DAcoVTIDenQ_DEO2_FDTD_TimeStepLinear,
   src/prop3DAcoVTIDenQ_DEO2_FDTD.h:356-376, the step the reference's propagateadjoint! runs): the
   adjoint dot-product test <F s, d> = <s, F* d> <= 1e-5 for seeded Gaussian source and receiver
@@ -20,7 +22,7 @@ The models follow the configs' construction (tests/helpers.py::config_model).
 import numpy as np
 import pytest
 
-from tests.helpers import config_model, host_view, smooth_noise_big
+from tests.helpers import config_model, host_view, smooth_noise_big, to_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +55,60 @@ def _weights(model, pts, shape, dt):
     v = model["v"][ix, iy, iz].astype(np.float64)
     b = model["b"][ix, iy, iz].astype(np.float64)
     return dt * dt * v * v / b
+
+
+@pytest.mark.timeout(900)  # the oracle's 2000 steps of 256^3 on the host: one to three minutes
+def test_c1_iso_256_forward_2000_steps_bit_exact(oracle):
+    """Iso 256^3 at 10 m, dt 1 ms, 2000 steps, Q = 100 at 10 Hz, 40-cell sponge, free surface; a
+    Hicks point source at 20 m depth in the middle of the grid with a 15 Hz Ricker wavelet, and a
+    receiver at 20 m depth on every (x, y) node (256 x 256).  The GPU runs the loop of
+    test/spacetime.jl:486-492 as TimeStepInjectExtractGraph; the oracle runs the same loop
+    (step, inject, extract) on the host.  Traces and final wavefields are bit-identical."""
+    import time
+    import wavefd_amd as w
+    n, d, dt, nt = 256, 10.0, 1e-3, 2000
+    model = config_model("iso", (n, n, n), d, dt, 1)
+    prop = w.Prop3DAcoIsoDenQ_DEO2_FDTD(nz=n, ny=n, nx=n, dz=d, dy=d, dx=d, dt=dt, nsponge=40,
+                                        freesurface=True, freqQ=10.0, qMin=0.1, qInterior=100.0)
+    orc = oracle.OracleIso(n, n, n, d, d, d, dt=dt, fs=True, nsponge=40, freqQ=10.0, qMin=0.1, qInterior=100.0,
+                           nthread=16)
+    for name, arr in model.items():
+        host_view(prop, name)[:] = arr.ravel()
+        getattr(orc, name)[...] = arr
+    del model
+    mid = (n - 1) * d / 2
+    src = w.hickscoeffs(F32(d), F32(d), F32(d), 0.0, 0.0, 0.0, n, n, n, [20.0], [mid], [mid])
+    wavelet = w.causal_ricker(15.0, dt * np.arange(nt))[None, :].astype(F32)
+    ix, iy = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    rcv = np.zeros(n * n, w.SourcePoint32)
+    rcv["iu"] = 2 + n * (iy.ravel() + n * ix.ravel()) + 1  # iz = 2: 20 m
+    rcv["ir"] = np.arange(n * n) + 1
+    rcv["c"] = 1.0
+    sparts, rparts = w.source_blocking(src, 16), w.receiver_blocking(rcv, 16)
+    w.coherence(prop, "device")
+    w.sync_to_device(prop)
+    w.set_sources(prop, sparts, wavelet, fields=1)
+    w.set_receivers(prop, rparts, nt)
+    t0 = time.time()
+    w.timestep_inject_extract_graph(prop, 1, nt)
+    got = w.receiver_data(prop, n * n, nt)
+    w.sync_to_host(prop)
+    p_gpu = {name: to_oracle(prop, name, (n, n, n)).copy() for name in ("pCur", "pOld")}
+    w.free(prop)
+    print(f"C1 Iso 256^3: {nt} GPU steps with injection and extraction in {time.time() - t0:.1f} s", flush=True)
+    spts, rpts = np.concatenate(sparts), np.concatenate(rparts)
+    traces = np.zeros((n * n, nt), F32)
+    t0 = time.time()
+    for it in range(1, nt + 1):
+        orc.time_step()
+        oracle.inject(orc.pCur, wavelet, it, spts)
+        oracle.extract(traces, orc.pCur, it, rpts)
+        if it % 250 == 0:
+            print(f"C1 oracle step {it} / {nt}, {time.time() - t0:.0f} s", flush=True)
+    assert np.abs(traces).max() > 0 and np.isfinite(traces).all()
+    assert np.array_equal(got, traces)
+    assert np.array_equal(p_gpu["pCur"], orc.pCur)
+    assert np.array_equal(p_gpu["pOld"], orc.pOld)
 
 
 def test_c2_vti_512_linear_step_dot_product():
