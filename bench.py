@@ -730,7 +730,7 @@ def _cpu_child(physics, n, target_s):
     """(child process) the oracle's two-pass step on the configs' n^3 model, FTZ/DAZ on"""
     host = _host_info()  # before OpenMP binds this thread to one place
     from oracle import oracle as orc
-    from tests.helpers import config_model, smooth_noise_big
+    from tests.helpers import config_angles, config_model, config_q, smooth_noise_big
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     cls = {"iso": orc.OracleIso, "vti": orc.OracleVTI, "tti": orc.OracleTTI}[physics]
@@ -746,23 +746,9 @@ def _cpu_child(physics, n, target_s):
     if physics == "tti":
         # tilt in [-45, 45] deg and azimuth in [0, 360) deg, smoothed by 25 cells (azimuth as a unit
         # vector); Q from 30 to 200 in the interior, the sponge keeping SetupDtOmegaInvQ's profile
-        th = (np.pi / 4) * smooth_noise_big((n, n, n), 25, 303)
-        prop.sinTheta[...] = np.sin(th)
-        prop.cosTheta[...] = np.cos(th)
-        del th
-        c = smooth_noise_big((n, n, n), 25, 304)
-        s = smooth_noise_big((n, n, n), 25, 305)
-        r = np.sqrt(c * c + s * s)
-        np.maximum(r, 1e-6, out=r)
-        prop.cosPhi[...] = c / r
-        prop.sinPhi[...] = s / r
-        del c, s, r
-        k = np.arange(n)
-        dist = np.minimum(k, n - 1 - k)
-        ks = np.minimum(np.minimum(dist[:, None, None], dist[None, :, None]), (n - 1 - k)[None, None, :])
-        q = (dt * 2 * np.pi * 10.0 / (115.0 + 85.0 * smooth_noise_big((n, n, n), 15, 306))).astype(np.float32)
-        np.copyto(prop.q, q, where=ks >= nsponge)
-        del q, ks
+        for name, arr in config_angles((n, n, n), 303).items():
+            getattr(prop, name)[...] = arr
+        config_q(prop.q, nsponge, dt, 306)
     p0 = smooth_noise_big((n, n, n), 4, 307)
     for name in ("pCur", "pOld") + (("mCur", "mOld") if physics != "iso" else ()):
         getattr(prop, name)[...] = p0

@@ -133,3 +133,32 @@ def config_model(physics, shape, d, dt, seed):
     rho = np.clip(0.31 * m["v"] ** 0.25, 1.0, 2.5)
     m["b"] = (1.0 / rho).astype(F32)
     return m
+
+
+def config_angles(shape, seed):
+    """The TTI workload's angles in oracle layout: tilt theta in [-45, 45] deg smoothed by 25 cells,
+    azimuth phi in [0, 360) deg as a unit vector (cos, sin) of two fields smoothed by 25 cells."""
+    th = (np.pi / 4) * smooth_noise_big(shape, 25, seed)
+    m = {"sinTheta": np.sin(th).astype(F32), "cosTheta": np.cos(th).astype(F32)}
+    del th
+    c = smooth_noise_big(shape, 25, seed + 1)
+    s = smooth_noise_big(shape, 25, seed + 2)
+    r = np.sqrt(c * c + s * s)
+    np.maximum(r, 1e-6, out=r)
+    m["cosPhi"] = (c / r).astype(F32)
+    m["sinPhi"] = (s / r).astype(F32)
+    return m
+
+
+def config_q(q, nsponge, dt, seed, fs=True):
+    """In place: Q from 30 to 200 in the interior (115 + 85 times noise smoothed by 15 cells, at
+    10 Hz: dtOmegaInvQ = dt 2 pi 10 / Q) where the sponge distance is at least nsponge; q keeps
+    SetupDtOmegaInvQ's profile elsewhere (distance as src/absorb.h:133-145)."""
+    nx, ny, nz = q.shape
+    dx_, dy_, kz = np.arange(nx), np.arange(ny), np.arange(nz)
+    dx_ = np.minimum(dx_, nx - 1 - dx_)
+    dy_ = np.minimum(dy_, ny - 1 - dy_)
+    dz_ = nz - 1 - kz if fs else np.minimum(kz, nz - 1 - kz)
+    ks = np.minimum(np.minimum(dx_[:, None, None], dy_[None, :, None]), dz_[None, None, :])
+    qi = (dt * 2 * np.pi * 10.0 / (115.0 + 85.0 * smooth_noise_big(q.shape, 15, seed))).astype(F32)
+    np.copyto(q, qi, where=ks >= nsponge)

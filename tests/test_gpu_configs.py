@@ -2,6 +2,8 @@
 
 * Iso 256^3 at 10 m, 2000 steps with a Hicks point source and 256 x 256 receivers at 20 m depth:
   traces and wavefields bit-identical to the oracle running the same loop on the host;
+* the TTI 800^3 workload's construction (tilt, azimuth, Q 30..200 as an array, 50-cell sponge) at
+  200^3 over 2000 steps with a Hicks source into P and M and 200 x 200 receivers: bit-identical;
 * VTI 512^3 at 12.5 m, 3000 steps of the linear step (Prop3DAcoVTIDenQ_DEO2_FDTD_TimeStepLinear,
   src/prop3DAcoVTIDenQ_DEO2_FDTD.h:356-376, the step the reference's propagateadjoint! runs): the
   adjoint dot-product test <F s, d> = <s, F* d> <= 1e-5 for seeded Gaussian source and receiver
@@ -109,6 +111,68 @@ def test_c1_iso_256_forward_2000_steps_bit_exact(oracle):
     assert np.array_equal(got, traces)
     assert np.array_equal(p_gpu["pCur"], orc.pCur)
     assert np.array_equal(p_gpu["pOld"], orc.pOld)
+
+
+@pytest.mark.timeout(900)  # the oracle's 2000 TTI steps of 200^3 on the host: about a minute
+def test_c3_construction_tti_200_forward_2000_steps_bit_exact(oracle):
+    """The TTI workload's model construction at 200^3 (12.5 m, dt 1 ms, 50-cell sponge, free
+    surface): VTI fields of the configs, tilt within +-45 deg and azimuth over 360 deg smoothed by
+    25 cells, Q from 30 to 200 in the interior as an array (the sponge keeps SetupDtOmegaInvQ's
+    profile).  A Hicks source at 25 m depth injected into P and M, a receiver at 25 m depth on
+    every (x, y) node; 2000 steps of TimeStepInjectExtractGraph against the oracle's loop of
+    test/spacetime.jl:486-492.  Traces and final wavefields are bit-identical."""
+    import time
+    import wavefd_amd as w
+    from tests.helpers import config_angles, config_q
+    n, d, dt, nt, nsponge = 200, 12.5, 1e-3, 2000, 50
+    shape = (n, n, n)
+    orc = oracle.OracleTTI(n, n, n, d, d, d, dt=dt, fs=True, nsponge=nsponge, freqQ=10.0, qMin=0.1,
+                           qInterior=100.0, nthread=16)
+    model = config_model("tti", shape, d, dt, 3)
+    model.update(config_angles(shape, 303))
+    config_q(orc.q, nsponge, dt, 306)
+    model["q"] = orc.q.copy()
+    prop = w.Prop3DAcoTTIDenQ_DEO2_FDTD(nz=n, ny=n, nx=n, dz=d, dy=d, dx=d, dt=dt, nsponge=nsponge,
+                                        freesurface=True, freqQ=10.0, qMin=0.1, qInterior=100.0)
+    for name, arr in model.items():
+        host_view(prop, name)[:] = arr.ravel()
+        getattr(orc, name)[...] = arr
+    del model
+    mid = (n - 1) * d / 2
+    src = w.hickscoeffs(F32(d), F32(d), F32(d), 0.0, 0.0, 0.0, n, n, n, [25.0], [mid + 3.1], [mid - 4.7])
+    wavelet = w.causal_ricker(15.0, dt * np.arange(nt))[None, :].astype(F32)
+    ix, iy = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    rcv = np.zeros(n * n, w.SourcePoint32)
+    rcv["iu"] = 2 + n * (iy.ravel() + n * ix.ravel()) + 1  # iz = 2: 25 m
+    rcv["ir"] = np.arange(n * n) + 1
+    rcv["c"] = 1.0
+    sparts, rparts = w.source_blocking(src, 16), w.receiver_blocking(rcv, 16)
+    w.coherence(prop, "device")
+    w.sync_to_device(prop)
+    w.set_sources(prop, sparts, wavelet, fields=3)
+    w.set_receivers(prop, rparts, nt)
+    t0 = time.time()
+    w.timestep_inject_extract_graph(prop, 1, nt)
+    got = w.receiver_data(prop, n * n, nt)
+    w.sync_to_host(prop)
+    gpu = {name: to_oracle(prop, name, shape).copy() for name in ("pCur", "pOld", "mCur", "mOld")}
+    w.free(prop)
+    print(f"C3 construction, TTI 200^3: {nt} GPU steps with injection and extraction in {time.time() - t0:.1f} s",
+          flush=True)
+    spts, rpts = np.concatenate(sparts), np.concatenate(rparts)
+    traces = np.zeros((n * n, nt), F32)
+    t0 = time.time()
+    for it in range(1, nt + 1):
+        orc.time_step()
+        oracle.inject(orc.pCur, wavelet, it, spts)
+        oracle.inject(orc.mCur, wavelet, it, spts)
+        oracle.extract(traces, orc.pCur, it, rpts)
+        if it % 250 == 0:
+            print(f"C3 construction oracle step {it} / {nt}, {time.time() - t0:.0f} s", flush=True)
+    assert np.abs(traces).max() > 0 and np.isfinite(traces).all()
+    assert np.array_equal(got, traces)
+    for name, arr in gpu.items():
+        assert np.array_equal(arr, getattr(orc, name)), name
 
 
 def test_c2_vti_512_linear_step_dot_product():
