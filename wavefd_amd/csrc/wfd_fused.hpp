@@ -151,11 +151,12 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<fu
     constexpr int NF = fused_nf(PHYS, EPI);
     static_assert(EPI != EPI_BORN_COUPLED || PHYS == ISO, "the coupled Born step is Iso's");
     constexpr int NM = PHYS == ISO ? 1 : (PHYS == VTI ? 4 : 8);  // material fields of the sandwich
-    // output operands: v, q, pOld (, mOld), then for the epilogues b and (BORN_INJ) dV, DP (, DM)
-    // or (IMAGE) g, DP (, DM)
-    constexpr int NEX = EPI == EPI_NONE ? 0 : (EPI == EPI_SAVE_DP ? 1 : (EPI == EPI_BORN_COUPLED ? 2 : 2 + NF));
+    // output operands: v, q, pOld (, mOld), then for the epilogues (COUPLED) dV, (BORN_INJ) dV,
+    // DP (, DM) or (IMAGE) g, DP (, DM).  The epilogues' b is the update's: the ring bq holds b
+    // at every written cell (the material is loaded there too)
+    constexpr int NEX = EPI == EPI_NONE || EPI == EPI_SAVE_DP ? 0 : (EPI == EPI_BORN_COUPLED ? 1 : 1 + NF);
     constexpr int NO = 2 + NF + NEX;
-    constexpr int IB = 2 + NF;  // index of b among the output operands
+    constexpr int IB = 2 + NF;  // index of the first epilogue operand
     constexpr int ZO = ZW - 7, YO = NROW - 7, LZ = ZW + 7;
     constexpr int NB = 7 * (LZ + NROW);  // border cells of the staged plane
     static_assert(NB <= ZW * NROW, "one border cell per thread at most");
@@ -176,7 +177,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<fu
     uint32_t own_sign;
     bool own_ok;
     const uint32_t own = col_off(y, z, ny, nz, fst, own_sign, own_ok);
-    const uint32_t moff = yzU ? own : 0u;  // material: only used inside the update region
+    // material: used inside the update region, and (the epilogues) its b at every written cell
+    const uint32_t moff = (yzU || (EPI != EPI_NONE && out_thread)) ? own : 0u;
     const uint32_t ooff = out_thread ? own : 0u;
     // this thread's border cell of the staged plane, if any
     uint32_t boff = 0, bsign = 0;
@@ -207,12 +209,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<fu
     oin[1] = A.q;
     oin[2] = A.pOld;
     if constexpr (NF == 2) oin[3] = A.mOld;
-    if constexpr (EPI != EPI_NONE) oin[IB] = A.b;
-    if constexpr (EPI == EPI_BORN_COUPLED) oin[IB + 1] = A.e_dv;
+    if constexpr (EPI == EPI_BORN_COUPLED) oin[IB] = A.e_dv;
     if constexpr (EPI == EPI_BORN_INJ || EPI == EPI_IMAGE) {
-        oin[IB + 1] = EPI == EPI_BORN_INJ ? A.e_dv : A.e_dp;
-        oin[IB + 2] = A.e_ip;
-        if constexpr (NF == 2) oin[IB + 3] = A.e_im;
+        oin[IB] = EPI == EPI_BORN_INJ ? A.e_dv : A.e_dp;
+        oin[IB + 1] = A.e_ip;
+        if constexpr (NF == 2) oin[IB + 2] = A.e_im;
     }
 
     const long nynz = g.nynz;
@@ -413,26 +414,26 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<fu
                 gst(pn_[0], A.pOld, x, nynz, so);
                 if constexpr (NF == 2) gst(pn_[1], A.mOld, x, nynz, so);
             }
-            const float sc = V * V / oc[IB];  // scaleSpatialDerivatives: pSpace *= v^2/b
+            const float sc = V * V / bq[sr];  // scaleSpatialDerivatives: pSpace *= v^2/b
             gst(L[0] * sc, A.e_dp, x, nynz, so);
             if constexpr (NF == 2) gst(L[1] * sc, A.e_dm, x, nynz, so);
         } else if constexpr (EPI == EPI_BORN_INJ) {
-            const float B = oc[IB], dv = oc[IB + 1];
+            const float B = bq[sr], dv = oc[IB];
             if constexpr (PHYS == ISO) {
                 const float dt2v2OverB = A.dt * A.dt * V * V / B;
                 const float factorV = 2 * B * dv / (V * V * V);
-                pn_[0] += dt2v2OverB * oc[IB + 2] * factorV;
+                pn_[0] += dt2v2OverB * oc[IB + 1] * factorV;
             } else {
                 const float factor = 2 * A.dt * A.dt * dv / V;
-                pn_[0] += factor * oc[IB + 2];
-                pn_[1] += factor * oc[IB + 3];
+                pn_[0] += factor * oc[IB + 1];
+                pn_[1] += factor * oc[IB + 2];
             }
             gst(pn_[0], A.pOld, x, nynz, so);
             if constexpr (NF == 2) gst(pn_[1], A.mOld, x, nynz, so);
         } else if constexpr (EPI == EPI_BORN_COUPLED) {
             // background: SAVE_DP's step; Born field: BORN_INJ's step with the DP just formed
             if (upd) gst(pn_[0], A.pOld, x, nynz, so);
-            const float B = oc[IB], dv = oc[IB + 1];
+            const float B = bq[sr], dv = oc[IB];
             const float sc = V * V / B;
             const float DP = L[0] * sc;
             gst(DP, A.e_dp, x, nynz, so);
@@ -445,14 +446,14 @@ __device__ __forceinline__ void fused_body(const FusedArgs<PHYS> &A, FusedLds<fu
                 gst(pn_[0], A.pOld, x, nynz, so);
                 if constexpr (NF == 2) gst(pn_[1], A.mOld, x, nynz, so);
             }
-            const float B = oc[IB];
-            float gk = oc[IB + 1];
+            const float B = bq[sr];
+            float gk = oc[IB];
             if constexpr (PHYS == ISO) {
                 const float factorV = +2 * B / (V * V * V);
-                gk += factorV * oc[IB + 2] * pq[0][0];
+                gk += factorV * oc[IB + 1] * pq[0][0];
             } else {
                 const float factor = 2 * B / (V * V * V);
-                gk += factor * (oc[IB + 2] * pq[0][0] + oc[IB + 3] * pq[1][0]);
+                gk += factor * (oc[IB + 1] * pq[0][0] + oc[IB + 2] * pq[1][0]);
             }
             gst(gk, A.e_dp, x, nynz, so);
         }

@@ -69,7 +69,9 @@ __device__ __forceinline__ void wide_body(const FusedArgs<PHYS> &A, WideLds<fuse
     constexpr int NF = fused_nf(PHYS, EPI);
     static_assert(EPI != EPI_BORN_COUPLED || PHYS == ISO, "the coupled Born step is Iso's");
     constexpr int NM = PHYS == ISO ? 1 : (PHYS == VTI ? 4 : 8);
-    constexpr int NEX = EPI == EPI_NONE ? 0 : (EPI == EPI_SAVE_DP ? 1 : (EPI == EPI_BORN_COUPLED ? 2 : 2 + NF));
+    // output operands as k_fused_step's: v, q, pOld (, mOld), then the epilogue's own; their b is
+    // the update's (bq holds b at every written cell)
+    constexpr int NEX = EPI == EPI_NONE || EPI == EPI_SAVE_DP ? 0 : (EPI == EPI_BORN_COUPLED ? 1 : 1 + NF);
     constexpr int NO = 2 + NF + NEX;
     constexpr int IB = 2 + NF;
     constexpr int ZW = 64, NROW = 8 * R, ZO = ZW - 7, YO = NROW - 7, LZ = ZW + 7;
@@ -107,7 +109,7 @@ __device__ __forceinline__ void wide_body(const FusedArgs<PHYS> &A, WideLds<fuse
         bool ok;
         own[i] = col_off(y, z, ny, nz, FST, own_sign, ok);
         rOk[i] = ok;
-        moff[i] = (yU[i] && zU) ? own[i] : 0u;
+        moff[i] = ((yU[i] && zU) || (EPI != EPI_NONE && rW[i] && zW)) ? own[i] : 0u;
         ooff[i] = (rW[i] && zW) ? own[i] : 0u;
     }
     bool anyOut = false, anyW = false;  // the wave holds an output (a written) row (uniform)
@@ -153,12 +155,11 @@ __device__ __forceinline__ void wide_body(const FusedArgs<PHYS> &A, WideLds<fuse
     oin[1] = A.q;
     oin[2] = A.pOld;
     if constexpr (NF == 2) oin[3] = A.mOld;
-    if constexpr (EPI != EPI_NONE) oin[IB] = A.b;
-    if constexpr (EPI == EPI_BORN_COUPLED) oin[IB + 1] = A.e_dv;
+    if constexpr (EPI == EPI_BORN_COUPLED) oin[IB] = A.e_dv;
     if constexpr (EPI == EPI_BORN_INJ || EPI == EPI_IMAGE) {
-        oin[IB + 1] = EPI == EPI_BORN_INJ ? A.e_dv : A.e_dp;
-        oin[IB + 2] = A.e_ip;
-        if constexpr (NF == 2) oin[IB + 3] = A.e_im;
+        oin[IB] = EPI == EPI_BORN_INJ ? A.e_dv : A.e_dp;
+        oin[IB + 1] = A.e_ip;
+        if constexpr (NF == 2) oin[IB + 2] = A.e_im;
     }
     const long nynz = g.nynz;
     constexpr bool QT = PHYS != TTI;
@@ -451,25 +452,25 @@ __device__ __forceinline__ void wide_body(const FusedArgs<PHYS> &A, WideLds<fuse
                         gst(pn_[0], A.pOld, x, nynz, so);
                         if constexpr (NF == 2) gst(pn_[1], A.mOld, x, nynz, so);
                     }
-                    const float sc = V * V / oc[IB][i];
+                    const float sc = V * V / bq[sr][i];
                     gst(L[0] * sc, A.e_dp, x, nynz, so);
                     if constexpr (NF == 2) gst(L[1] * sc, A.e_dm, x, nynz, so);
                 } else if constexpr (EPI == EPI_BORN_INJ) {
-                    const float B = oc[IB][i], dv = oc[IB + 1][i];
+                    const float B = bq[sr][i], dv = oc[IB][i];
                     if constexpr (PHYS == ISO) {
                         const float dt2v2OverB = A.dt * A.dt * V * V / B;
                         const float factorV = 2 * B * dv / (V * V * V);
-                        pn_[0] += dt2v2OverB * oc[IB + 2][i] * factorV;
+                        pn_[0] += dt2v2OverB * oc[IB + 1][i] * factorV;
                     } else {
                         const float factor = 2 * A.dt * A.dt * dv / V;
-                        pn_[0] += factor * oc[IB + 2][i];
-                        pn_[1] += factor * oc[IB + 3][i];
+                        pn_[0] += factor * oc[IB + 1][i];
+                        pn_[1] += factor * oc[IB + 2][i];
                     }
                     gst(pn_[0], A.pOld, x, nynz, so);
                     if constexpr (NF == 2) gst(pn_[1], A.mOld, x, nynz, so);
                 } else if constexpr (EPI == EPI_BORN_COUPLED) {
                     if (upd) gst(pn_[0], A.pOld, x, nynz, so);
-                    const float B = oc[IB][i], dv = oc[IB + 1][i];
+                    const float B = bq[sr][i], dv = oc[IB][i];
                     const float sc = V * V / B;
                     const float DP = L[0] * sc;
                     gst(DP, A.e_dp, x, nynz, so);
@@ -482,14 +483,14 @@ __device__ __forceinline__ void wide_body(const FusedArgs<PHYS> &A, WideLds<fuse
                         gst(pn_[0], A.pOld, x, nynz, so);
                         if constexpr (NF == 2) gst(pn_[1], A.mOld, x, nynz, so);
                     }
-                    const float B = oc[IB][i];
-                    float gk = oc[IB + 1][i];
+                    const float B = bq[sr][i];
+                    float gk = oc[IB][i];
                     if constexpr (PHYS == ISO) {
                         const float factorV = +2 * B / (V * V * V);
-                        gk += factorV * oc[IB + 2][i] * pq[0][i][qs(0)];
+                        gk += factorV * oc[IB + 1][i] * pq[0][i][qs(0)];
                     } else {
                         const float factor = 2 * B / (V * V * V);
-                        gk += factor * (oc[IB + 2][i] * pq[0][i][qs(0)] + oc[IB + 3][i] * pq[1][i][qs(0)]);
+                        gk += factor * (oc[IB + 1][i] * pq[0][i][qs(0)] + oc[IB + 2][i] * pq[1][i][qs(0)]);
                     }
                     gst(gk, A.e_dp, x, nynz, so);
                 }
