@@ -148,8 +148,8 @@ def test_driver_fused_equals_separate(kind):
                          spacetime.causal_ricker(25.0, 1e-3 * np.arange(nt)))
     w = ginsu.Window((nz, ny, nx), (0, 0, 0), (nz, ny, nx))
     res = []
-    modes = [(True, None), (False, None)] + ([(True, False)] if kind == "iso" else [])
-    for fused, coupled in modes:  # Iso: coupled passes (default), separate calls, fused uncoupled
+    modes = [(True, None), (False, None)] + ([(True, True)] if kind == "iso" else [])
+    for fused, coupled in modes:  # fused passes (the default with exact storage), separate calls, Iso coupled
         img = ginsu.DeviceBuffer(dv.nbytes)
         img.zero()
         out = born.born_shot(kind, w, (10.0, 10.0, 10.0), 1e-3, shot, {k: q.ptr for k, q in bufs.items()}, dvb.ptr,

@@ -903,8 +903,11 @@ def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=
     across shots.  adjoint_data: (nr, nt) float32 data for the adjoint pass (default: the Born
     data of pass 2, i.e. image = J^T J dv).  fused: each pass's per-step pair as one fused step
     (TimeStepSaveDP, TimeStepBornInjection_V, TimeStepAdjointBornAccumulation_V; the same
-    results as the separate calls).  coupled (default: Iso with fused): passes 1 and 2 as one
-    pass over both wavefields (TimeStepBornCoupled_V; the same results again).  snapshot_codec: a
+    results as the separate calls).  coupled: passes 1 and 2 as one pass over both wavefields
+    (TimeStepBornCoupled_V; the same results again); by default for Iso with fused steps whose
+    snapshots are not stored exactly, since it forms the Born source from the exact DP inside the
+    step (a lossy store then affects the imaging only).  With exact storage the two one-wavefield
+    passes, on the two-workgroups-per-CU tile, are faster.  snapshot_codec: a
     ``lcodec.LearnedCodec`` to keep DP (DM) as learned-codec codes instead of exactly (decoded
     by its network, or by the plain dequantisation with learned=False).  checkpoint: an interval k
     to recompute DP (DM) from checkpoints of the background every k steps instead of storing them
@@ -918,7 +921,8 @@ def born_shot(physics, window, spacing, dt, shot, model, dmodel, image, nsponge=
     if physics not in _CLS:
         raise ValueError("physics must be 'iso' or 'vti'")
     if coupled is None:
-        coupled = fused and physics == "iso"
+        coupled = fused and physics == "iso" and not (checkpoint is None and predictive is None and
+                                                      snapshot_codec is None)
     t0 = time.perf_counter()
     run = _ShotRun(physics, window, spacing, dt, shot, model, dmodel, nsponge, freesurface, freqQ, qMin, qInterior,
                    snapshots, snapshot_codec, learned, checkpoint, fused, predictive, residual_codec)
@@ -1004,6 +1008,7 @@ def _run_groups(physics, windows, spacing, dt, shots, model, dmodel, image, nspo
                                      predictive=predictive))
             _run_passes(runs, fused, None if adjoint_data is None else [adjoint_data[i] for i in g], keep_born_data,
                         coupled=fused and physics == "iso" and
+                        not all(isinstance(r.store, _ExactStore) for r in runs) and
                         (len(runs) == 1 or not any(isinstance(r.store, _CheckpointStore) for r in runs)))
             for i, r in zip(g, runs):
                 r.finish(image)
