@@ -81,8 +81,8 @@ def dminus(g, inX, inZ):
     return ox, oz
 
 
-def setup_q(fs, nx, nz, nsponge, dt, freqQ, qMin, qInterior):
-    q = np.zeros((nx, nz), F32)
+def setup_q(fs, nx, nz, nsponge, dt, freqQ, qMin, qInterior, out=None):
+    q = np.zeros((nx, nz), F32) if out is None else out
     if lib().wo2_setup_q(1 if fs else 0, nx, nz, nsponge, F32(dt), F32(freqQ), F32(qMin), F32(qInterior), _p(q)):
         raise ValueError("freqQ too small")
     return q

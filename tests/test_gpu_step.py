@@ -58,6 +58,31 @@ def test_q_profile_matches_oracle(oracle, kind, fs):
 
 @pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
 @pytest.mark.parametrize("fs", [False, True])
+def test_unphysical_q(oracle, kind, fs):
+    """qMin = qInterior = 0 (src/absorb.h:105-109, test/absorb.jl:85-99): a new handle's Q is all
+    zero; set up again over a profile, only the first two values are cleared, as the reference's
+    memset of sizeof(float *) bytes does -- the oracle's restatement of it, pinned against the
+    reference's own absorb.h.  The step then reads that array."""
+    import wavefd_amd as w
+    shape = (21, 19, 23)
+    nx, ny, nz = shape
+    cls = {"iso": w.Prop3DAcoIsoDenQ_DEO2_FDTD, "vti": w.Prop3DAcoVTIDenQ_DEO2_FDTD,
+           "tti": w.Prop3DAcoTTIDenQ_DEO2_FDTD}[kind]
+    prop = cls(nz=nz, ny=ny, nx=nx, dz=8.0, dy=12.5, dx=10.0, dt=1e-3, nsponge=5, freesurface=fs, freqQ=10.0,
+               qMin=0.0, qInterior=0.0)
+    assert not to_oracle(prop, "q", shape).any()
+    prop._lib.SetupDtOmegaInvQ(prop.p, 10.0, 0.5, 80.0)
+    want = oracle.setup_q(fs, nx, ny, nz, 5, 1e-3, 10.0, 0.5, 80.0)
+    assert np.array_equal(to_oracle(prop, "q", shape), want)
+    prop._lib.SetupDtOmegaInvQ(prop.p, 10.0, 0.0, 0.0)
+    oracle.setup_q(fs, nx, ny, nz, 5, 1e-3, 10.0, 0.0, 0.0, out=want)
+    got = to_oracle(prop, "q", shape)
+    assert np.array_equal(got, want) and not got.ravel()[:2].any() and got.ravel()[2:].all()
+    w.free(prop)
+
+
+@pytest.mark.parametrize("kind", ["iso", "vti", "tti"])
+@pytest.mark.parametrize("fs", [False, True])
 @pytest.mark.parametrize("fused", ["64x16", "32x32", "wide", "wide2", None])
 @pytest.mark.parametrize("shape", SHAPES)
 def test_steps_bit_exact(oracle, kind, fs, fused, shape):

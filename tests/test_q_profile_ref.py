@@ -81,3 +81,20 @@ def test_2d_ramp_uses_the_float_log(ref):
                                                                       np.float64(np.log(F32(qmin))))
     qprof_float = np.exp(lq_f.astype(F32))
     assert not np.array_equal(qprof_double, qprof_float)
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_unphysical_q_clears_what_the_reference_clears(ref, fs):
+    """qMin, qInterior < FLT_EPSILON: the reference prints a warning and clears sizeof(float *)
+    bytes, the array's first two values (src/absorb.h:44,107), leaving the rest as it was; a freshly
+    allocated (zeroed) array therefore stays all zero, the case test/absorb.jl:85-99 checks."""
+    nx, ny, nz = 13, 11, 15
+    want = np.ones((nx, ny, nz), F32)
+    ref.ref_setup_q_3d(int(fs), nx, ny, nz, 5, F32(1e-3), F32(10.0), F32(0.0), F32(0.0), _p(want))
+    got = o3.setup_q(fs, nx, ny, nz, 5, 1e-3, 10.0, 0.0, 0.0, out=np.ones((nx, ny, nz), F32))
+    assert np.array_equal(got, want)
+    assert not want.ravel()[:2].any() and (want.ravel()[2:] == 1).all()
+    want2 = np.ones((nx, nz), F32)
+    ref.ref_setup_q_2d(int(fs), nx, nz, 5, F32(1e-3), F32(10.0), F32(0.0), F32(0.0), _p(want2))
+    got2 = o2.setup_q(fs, nx, nz, 5, 1e-3, 10.0, 0.0, 0.0, out=np.ones((nx, nz), F32))
+    assert np.array_equal(got2, want2)
