@@ -548,20 +548,26 @@ __global__ void __launch_bounds__(WIDE_THREADS, wide_wpe(fused_nf(PHYS, EPI), R)
 }
 
 // x-planes per chunk of the wide tile.  Where the plain VTI / TTI step keeps rounds in step
-// (fused_lockstep_of), the shortest chunks of at least 64 planes that give the 20 rounds it needs:
-// each chunk re-synchronises the workgroups of a round, so neighbouring tiles keep reading their
-// shared lines while these are in the XCD's L2 (measured, TTI 800^3: 100-plane chunks in step
-// 12.39 ms, one 800-plane chunk 12.78 ms).  Otherwise the round-balancing rule of fused_cx.
+// (fused_lockstep_of), chunks of at least 64 planes that give 25 rounds, else the shortest that give
+// the 20 rounds it needs: each chunk re-synchronises the workgroups of a round, so neighbouring
+// tiles keep reading their shared lines while these are in the XCD's L2 (measured, TTI 800^3:
+// 100-plane chunks in step 12.39 ms, one 800-plane chunk 12.78 ms; with the current kernel 10
+// chunks of 80 planes, 25.7 rounds, 11.13 against 11.35 ms for 8 of 100, and VTI 800^3 8.52
+// against 8.79 ms; 12 or more chunks lose; at 704^3 and 960^3 the choice moves nothing beyond
+// 1 %).  Otherwise the round-balancing rule of fused_cx.
 static inline int wide_cx(const Prop *p, long tiles, int phys, int epi) {
     if (p->fused_chunk > 0 || std::getenv("WAVEFD_FUSED_CX")) return fused_cx(p, tiles, 8);
     const long cus = p->cu_count > 0 ? p->cu_count : 256;
     const bool auto_ls = p->fused_lockstep < 0 && !std::getenv("WAVEFD_FUSED_LOCKSTEP");
     if (phys != ISO && epi == EPI_NONE && cus % 8 == 0 && (auto_ls || p->fused_lockstep == 1)) {
+        long c20 = 0;
         for (long nxc = 1; nxc <= 64 && nxc <= p->nx; nxc++) {
             const long cx = (p->nx + nxc - 1) / nxc;
             if (cx < 64) break;
-            if (tiles * nxc >= 20L * cus) return (int)cx;
+            if (c20 == 0 && tiles * nxc >= 20L * cus) c20 = cx;
+            if (tiles * nxc >= 25L * cus) return (int)cx;
         }
+        if (c20 > 0) return (int)c20;
     }
     return fused_cx(p, tiles, 8);
 }
