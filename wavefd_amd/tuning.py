@@ -84,12 +84,13 @@ def set_fused_config(prop, tile="auto", lockstep=None, chunk=0):
 
 def _candidates(prop):
     """Distinct effective configurations: every tile, rounds in step on and off, and the
-    x-chunk lengths that split nx into 1, 2, 3, 4, 6 or 8 chunks besides the automatic one."""
+    x-chunk lengths that split nx into 1, 2, 3, 4, 5, 6, 8, 10 or 12 chunks besides the automatic
+    one (rounds in step take effect only where a split gives 20 rounds of workgroups)."""
     nx = prop.shape[2]
     seen, out = set(), []
     for tile in ("64x16", "32x32", "wide", "wide2"):
         for lockstep in (False, True):
-            for nxc in (0, 1, 2, 3, 4, 6, 8):
+            for nxc in (0, 1, 2, 3, 4, 5, 6, 8, 10, 12):
                 chunk = 0 if nxc == 0 else -(-nx // nxc)
                 set_fused_config(prop, tile, lockstep, chunk)
                 eff = fused_config(prop)
