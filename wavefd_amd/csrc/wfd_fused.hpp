@@ -48,6 +48,8 @@ struct FusedArgs {
     int nxc;       // x-chunks
     int bz, by;    // tiles per block of the XCD-local order (bz = 0: plain z-fastest, -1: y-fastest)
     int per_xcd;     // workgroups per XCD that run at once (rounds kept in step); 0: no waiting
+    int slack;       // a round starts when all but this many workgroups of the earlier rounds finished
+                     // (WAVEFD_FUSED_SLACK sets it, for measurements)
     unsigned *sync;  // per-XCD count of finished workgroups, zeroed before the launch
     const float *pCur, *mCur;
     float *pOld, *mOld, *pSpace, *mSpace;
@@ -703,6 +705,8 @@ static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float 
     A.sP = f[F_SP].d;
     A.cP = f[F_CP].d;
     A.per_xcd = 0;
+    A.slack = -1;  // the launch's default
+    if (const char *e = std::getenv("WAVEFD_FUSED_SLACK")) A.slack = std::max(0, std::atoi(e));
     A.sync = p->fused_sync;
     return (long)A.nzt * A.nyt * nxc;
 }
@@ -734,6 +738,7 @@ static inline bool launch_fused(Prop *p, bool linear, float *e0 = nullptr, float
     const bool lockstep = fused_lockstep_of(p, PHYS, EPI, ntiles);
     if (lockstep && p->fused_sync) {
         A.per_xcd = p->cu_count / 8;
+        if (A.slack < 0) A.slack = 0;
         // a kernel, not a memset, so that the step can be captured in a graph
         k_zero_u32<<<1, 64, 0, p->stream>>>(p->fused_sync, 8);
         WFD_LAUNCHED();

@@ -528,8 +528,10 @@ __global__ void __launch_bounds__(WIDE_THREADS, wide_wpe(fused_nf(PHYS, EPI), R)
     const int per = A.per_xcd;
     if (per > 0 && loc >= per) {
         // rounds in step, as k_fused_step: wait (bounded) until the XCD finished every earlier round
+        // but `slack` workgroups
         if (threadIdx.x == 0) {
-            const unsigned target = (unsigned)(loc / per * per);
+            const int t0 = loc / per * per - A.slack;
+            const unsigned target = (unsigned)(t0 > 0 ? t0 : 0);
             for (int i = 0; i < 4096; i++) {
                 if (__hip_atomic_load(&A.sync[xcd], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
                 __builtin_amdgcn_s_sleep(16);
@@ -584,6 +586,9 @@ static inline bool launch_wide(Prop *p, bool linear, float *e0, float *e1, float
     const bool lockstep = fused_lockstep_of(p, PHYS, EPI, ntiles);
     if (lockstep && p->fused_sync) {
         A.per_xcd = p->cu_count / 8;
+        // a quarter of a round of slack: TTI 800^3 11.02 against 11.09 ms for the strict barrier,
+        // VTI 800^3 8.39 against 8.51 ms (one round of slack: 11.52 and 8.46 ms)
+        if (A.slack < 0) A.slack = A.per_xcd / 4;
         k_zero_u32<<<1, 64, 0, p->stream>>>(p->fused_sync, 8);
         WFD_LAUNCHED();
     }
