@@ -49,6 +49,7 @@ struct FusedArgs {
     int bz, by;    // tiles per block of the XCD-local order (bz = 0: plain z-fastest, -1: y-fastest)
     int per_xcd;     // workgroups per XCD that run at once (rounds kept in step); 0: no waiting
     int slack;       // a round starts when all but this many workgroups of the earlier rounds finished
+    int xmap;        // 0: each XCD a contiguous run of tiles; G > 0: groups of G tiles dealt to the XCDs in turn
                      // (WAVEFD_FUSED_SLACK sets it, for measurements)
     unsigned *sync;  // per-XCD count of finished workgroups, zeroed before the launch
     const float *pCur, *mCur;
@@ -513,6 +514,17 @@ __device__ __forceinline__ bool fused_tile_of_wg(const FusedArgs<PHYS> &A, int b
                                                  int &xc) {
     const int xcd = bid & 7;
     const int ntiles = A.nzt * A.nyt * A.nxc;
+    if (A.xmap > 0) {
+        // groups of G consecutive tiles dealt to the XCDs in turn: the eight XCDs' rounds run
+        // neighbouring groups at the same time; ids past the last tile do nothing
+        const int G = A.xmap;
+        loc = bid >> 3;
+        const int sr = loc / G;
+        const int tile = (sr * 8 + xcd) * G + (loc - sr * G);
+        if (tile >= ntiles) return false;
+        tile_coords(A, tile, zt, yt, xc);
+        return true;
+    }
     const int q8 = ntiles >> 3, r8 = ntiles & 7;
     const int start = xcd < r8 ? xcd * (q8 + 1) : r8 * (q8 + 1) + (xcd - r8) * q8;
     const int count = q8 + (xcd < r8 ? 1 : 0);
@@ -705,6 +717,7 @@ static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float 
     A.sP = f[F_SP].d;
     A.cP = f[F_CP].d;
     A.per_xcd = 0;
+    A.xmap = 0;
     A.slack = -1;  // the launch's default
     if (const char *e = std::getenv("WAVEFD_FUSED_SLACK")) A.slack = std::max(0, std::atoi(e));
     A.sync = p->fused_sync;

@@ -550,26 +550,21 @@ __global__ void __launch_bounds__(WIDE_THREADS, wide_wpe(fused_nf(PHYS, EPI), R)
 }
 
 // x-planes per chunk of the wide tile.  Where the plain VTI / TTI step keeps rounds in step
-// (fused_lockstep_of), chunks of at least 64 planes that give 25 rounds, else the shortest that give
-// the 20 rounds it needs: each chunk re-synchronises the workgroups of a round, so neighbouring
-// tiles keep reading their shared lines while these are in the XCD's L2 (measured, TTI 800^3:
-// 100-plane chunks in step 12.39 ms, one 800-plane chunk 12.78 ms; with the current kernel 10
-// chunks of 80 planes, 25.7 rounds, 11.13 against 11.35 ms for 8 of 100, and VTI 800^3 8.52
-// against 8.79 ms; 12 or more chunks lose; at 704^3 and 960^3 the choice moves nothing beyond
-// 1 %).  Otherwise the round-balancing rule of fused_cx.
+// (fused_lockstep_of), the shortest chunks of at least 64 planes that give the 20 rounds it needs:
+// each chunk re-synchronises the workgroups of a round, so neighbouring tiles keep reading their
+// shared lines while these are in the XCD's L2 (measured, TTI 800^3 with the XCDs taking turns: 8
+// chunks of 100 planes 10.40 ms, 10 of 80 10.45, 9 of 89 10.65; 6 or 7 chunks, too few rounds to
+// keep in step, 11.0-11.4).  Otherwise the round-balancing rule of fused_cx.
 static inline int wide_cx(const Prop *p, long tiles, int phys, int epi) {
     if (p->fused_chunk > 0 || std::getenv("WAVEFD_FUSED_CX")) return fused_cx(p, tiles, 8);
     const long cus = p->cu_count > 0 ? p->cu_count : 256;
     const bool auto_ls = p->fused_lockstep < 0 && !std::getenv("WAVEFD_FUSED_LOCKSTEP");
     if (phys != ISO && epi == EPI_NONE && cus % 8 == 0 && (auto_ls || p->fused_lockstep == 1)) {
-        long c20 = 0;
         for (long nxc = 1; nxc <= 64 && nxc <= p->nx; nxc++) {
             const long cx = (p->nx + nxc - 1) / nxc;
             if (cx < 64) break;
-            if (c20 == 0 && tiles * nxc >= 20L * cus) c20 = cx;
-            if (tiles * nxc >= 25L * cus) return (int)cx;
+            if (tiles * nxc >= 20L * cus) return (int)cx;
         }
-        if (c20 > 0) return (int)c20;
     }
     return fused_cx(p, tiles, 8);
 }
@@ -586,13 +581,28 @@ static inline bool launch_wide(Prop *p, bool linear, float *e0, float *e1, float
     const bool lockstep = fused_lockstep_of(p, PHYS, EPI, ntiles);
     if (lockstep && p->fused_sync) {
         A.per_xcd = p->cu_count / 8;
-        // a quarter of a round of slack: TTI 800^3 11.02 against 11.09 ms for the strict barrier,
-        // VTI 800^3 8.39 against 8.51 ms (one round of slack: 11.52 and 8.46 ms)
+        // a quarter of a round of slack (with contiguous runs per XCD: TTI 800^3 11.02 against 11.09
+        // ms for the strict barrier, VTI 800^3 8.39 against 8.51 ms; with the XCDs taking turns
+        // 10.40 against 10.44 ms)
         if (A.slack < 0) A.slack = A.per_xcd / 4;
         k_zero_u32<<<1, 64, 0, p->stream>>>(p->fused_sync, 8);
         WFD_LAUNCHED();
     }
-    k_wide_step<PHYS, EPI, R><<<dim3((unsigned)ntiles), dim3(WIDE_THREADS), 0, p->stream>>>(A);
+    // the XCDs take turns: group i of G tiles (the workgroups one round of an XCD holds) goes to
+    // XCD i mod 8, so the eight XCDs run neighbouring groups at the same time and share their halo
+    // lines through the memory-side cache, where contiguous runs per XCD put them chunks apart
+    // (TTI 800^3 10.40 against 11.05 ms, VTI 512^3 2.28 against 2.39, TTI 512^3 3.17 against 3.21,
+    // Iso 512^3 1.06 against 1.09, Iso 256^3 unchanged; WAVEFD_FUSED_XMAP=0 keeps the runs, for
+    // measurements).  The grid covers the deals: XCD 0 holds the most workgroups, those of every
+    // full deal and of the last one's first group; ids past the last tile do nothing.
+    long grid = ntiles;
+    const char *xm = std::getenv("WAVEFD_FUSED_XMAP");
+    if (p->cu_count % 8 == 0 && !(xm && std::atoi(xm) == 0)) {
+        const long G = (long)(p->cu_count / 8) * (two_per_cu ? 2 : 1), deals = ntiles / (8 * G);
+        A.xmap = (int)G;
+        grid = 8 * (deals * G + std::min(G, ntiles - deals * 8 * G));
+    }
+    k_wide_step<PHYS, EPI, R><<<dim3((unsigned)grid), dim3(WIDE_THREADS), 0, p->stream>>>(A);
     return true;
 }
 
