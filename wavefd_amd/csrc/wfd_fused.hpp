@@ -725,9 +725,10 @@ static inline long fill_fused(Prop *p, bool linear, float *e0, float *e1, float 
 }
 
 // rounds in step: the automatic rule, overridden by the handle's setting, else by the
-// environment; never where the rounds cannot be formed
-static inline bool fused_lockstep_of(const Prop *p, int phys, int epi, long ntiles) {
-    const bool can = p->cu_count % 8 == 0 && ntiles >= 20L * p->cu_count;
+// environment; never where the rounds cannot be formed (min_rounds of one workgroup per CU: 20
+// for k_fused_step, 12 for the wide tile, whose XCDs take turns)
+static inline bool fused_lockstep_of(const Prop *p, int phys, int epi, long ntiles, long min_rounds = 20) {
+    const bool can = p->cu_count % 8 == 0 && ntiles >= min_rounds * p->cu_count;
     bool lockstep = can && phys != ISO && epi == EPI_NONE;
     if (p->fused_lockstep >= 0)
         lockstep = can && p->fused_lockstep != 0;
@@ -837,7 +838,7 @@ static inline bool get_fused_config(const Prop *p, long *out) {
                                           : fused_cx(p, tiles, 4);
     const long ntiles = tiles * ((p->nx + cx - 1) / cx);
     out[0] = tile;
-    out[1] = fused_lockstep_of(p, p->phys, EPI_NONE, ntiles) ? 1 : 0;
+    out[1] = fused_lockstep_of(p, p->phys, EPI_NONE, ntiles, tile == TILE_64x32W ? WIDE_LOCKSTEP_ROUNDS : 20) ? 1 : 0;
     out[2] = cx;
     out[3] = ntiles;
     return true;

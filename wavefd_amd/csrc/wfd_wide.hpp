@@ -550,21 +550,36 @@ __global__ void __launch_bounds__(WIDE_THREADS, wide_wpe(fused_nf(PHYS, EPI), R)
 }
 
 // x-planes per chunk of the wide tile.  Where the plain VTI / TTI step keeps rounds in step
-// (fused_lockstep_of), the shortest chunks of at least 64 planes that give the 20 rounds it needs:
-// each chunk re-synchronises the workgroups of a round, so neighbouring tiles keep reading their
-// shared lines while these are in the XCD's L2 (measured, TTI 800^3 with the XCDs taking turns: 8
-// chunks of 100 planes 10.40 ms, 10 of 80 10.45, 9 of 89 10.65; 6 or 7 chunks, too few rounds to
-// keep in step, 11.0-11.4).  Otherwise the round-balancing rule of fused_cx.
+// (fused_lockstep_of, from WIDE_LOCKSTEP_ROUNDS rounds), chunks of at least 64 planes (TTI) or 96
+// (VTI, whose cheaper planes leave the warm-up a larger share) balanced as
+// fused_cx balances them (whole rounds against the warm-up each chunk pays) among the splits that
+// give those rounds: each chunk re-synchronises the workgroups of a round, so neighbouring tiles
+// keep reading their shared lines while these are in the XCD's L2.  Measured with the XCDs taking
+// turns (one process): TTI 800^3 5 chunks of 160 planes (12.9 rounds) 10.25-10.28 ms, 7 of 115
+// 10.21, 8 of 100 10.38, 6 of 134 10.48, 4 of 200 10.73; VTI 800^3 7.88, 7.94, 8.10, 8.16 and
+// 8.35 ms -- the order of that balance.  Where 64-plane chunks alone give the rounds: TTI 640^3
+// 5.83-5.92 ms in step against 5.94-6.06 for 128 planes without, VTI 640^3 4.59 against 4.40-4.42.
+// Otherwise fused_cx without rounds in step.
+constexpr long WIDE_LOCKSTEP_ROUNDS = 12;
 static inline int wide_cx(const Prop *p, long tiles, int phys, int epi) {
     if (p->fused_chunk > 0 || std::getenv("WAVEFD_FUSED_CX")) return fused_cx(p, tiles, 8);
     const long cus = p->cu_count > 0 ? p->cu_count : 256;
     const bool auto_ls = p->fused_lockstep < 0 && !std::getenv("WAVEFD_FUSED_LOCKSTEP");
     if (phys != ISO && epi == EPI_NONE && cus % 8 == 0 && (auto_ls || p->fused_lockstep == 1)) {
+        const long min_cx = phys == TTI ? 64 : 96;
+        long best_cx = 0;
+        double best = 1e300;
         for (long nxc = 1; nxc <= 64 && nxc <= p->nx; nxc++) {
             const long cx = (p->nx + nxc - 1) / nxc;
-            if (cx < 64) break;
-            if (tiles * nxc >= 20L * cus) return (int)cx;
+            if (cx < min_cx) break;
+            if ((p->nx + cx - 1) / cx != nxc || tiles * nxc < WIDE_LOCKSTEP_ROUNDS * cus) continue;
+            const double cost = (double)((tiles * nxc + cus - 1) / cus) * (double)chunk_steps(p->nx, nxc, 8);
+            if (cost < best * 0.999) {
+                best = cost;
+                best_cx = cx;
+            }
         }
+        if (best_cx > 0) return (int)best_cx;
     }
     return fused_cx(p, tiles, 8);
 }
@@ -578,7 +593,7 @@ static inline bool launch_wide(Prop *p, bool linear, float *e0, float *e1, float
     constexpr bool two_per_cu = wide_wpe(fused_nf(PHYS, EPI), R) == 4;
     const long ntiles =
         fill_fused<PHYS, 64, 8 * R, EPI>(p, linear, e0, e1, e2, A, 8, two_per_cu ? wide_cx2 : wide_cx);
-    const bool lockstep = fused_lockstep_of(p, PHYS, EPI, ntiles);
+    const bool lockstep = fused_lockstep_of(p, PHYS, EPI, ntiles, WIDE_LOCKSTEP_ROUNDS);
     if (lockstep && p->fused_sync) {
         A.per_xcd = p->cu_count / 8;
         // a quarter of a round of slack (with contiguous runs per XCD: TTI 800^3 11.02 against 11.09
