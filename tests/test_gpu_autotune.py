@@ -58,7 +58,7 @@ def test_results_do_not_depend_on_the_configuration(kind):
 @pytest.mark.parametrize("kind", ["vti", "tti"])
 @pytest.mark.parametrize("tile,chunk", [("64x16", 2), ("wide", 1)])
 def test_rounds_in_step_change_no_result(kind, tile, chunk):
-    # enough workgroups for 20 rounds of one per CU (5120 on 256 CUs): 45 tiles per plane
+    # enough workgroups for 20 rounds of one per CU (5120 on 256 CUs; 12 rounds on the wide tile): 45 tiles per plane
     # (128 x 128, 64 x 16 tile) times 128 chunks of 2 x-planes
     shape = (256, 128, 128)
     # (wide tile: 24 tiles per plane times 256 chunks of one x-plane)

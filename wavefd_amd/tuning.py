@@ -85,7 +85,8 @@ def set_fused_config(prop, tile="auto", lockstep=None, chunk=0):
 def _candidates(prop):
     """Distinct effective configurations: every tile, rounds in step on and off, and the
     x-chunk lengths that split nx into 1, 2, 3, 4, 5, 6, 8, 10 or 12 chunks besides the automatic
-    one (rounds in step take effect only where a split gives 20 rounds of workgroups)."""
+    one (rounds in step take effect only where a split gives 20 rounds of workgroups, 12 on the wide
+    tile)."""
     nx = prop.shape[2]
     seen, out = set(), []
     for tile in ("64x16", "32x32", "wide", "wide2"):
