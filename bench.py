@@ -475,7 +475,7 @@ PREDICTIVE = (48, 3, 16)
 
 def run_store(physics="vti", n=512, nt=400, seed=5, k=96, levels=(5, 11)):
     """The Born gradient with compressed snapshots judged against exact storage: one shot on an
-    n^3 VTI model (the configs' construction) at 12.5 m, dt 1 ms, nt steps, a +-2 % smoothed dv;
+    n^3 VTI model (the benchmark model's construction) at 12.5 m, dt 1 ms, nt steps, a +-2 % smoothed dv;
     the image with the wave-predicted store (born.py::_WaveStore, segments of k frames, 16-bit
     keyframes, residuals quantised to each of `levels`) against the image with exact snapshots (the
     checkpointed store, which gives the exact store's image bit for bit).  Returns the stored-byte
@@ -727,7 +727,7 @@ def _host_info():
 
 
 def _cpu_child(physics, n, target_s):
-    """(child process) the oracle's two-pass step on the configs' n^3 model, FTZ/DAZ on"""
+    """(child process) the oracle's two-pass step on the benchmark's n^3 model, FTZ/DAZ on"""
     host = _host_info()  # before OpenMP binds this thread to one place
     from oracle import oracle as orc
     from tests.helpers import config_angles, config_model, config_q, smooth_noise_big
@@ -766,7 +766,7 @@ def _cpu_child(physics, n, target_s):
     el = time.perf_counter() - t0
     return {"value": n ** 3 * k / el / 1e6, "unit": "Mcells/s", "cores": threads, "kind": "port",
             "sample": "oracle two-pass %s step (the reference's structure, nbx = nby = 8, nbz = nz) on the "
-                      "configs' %d^3 model, %d steps in %.1f s after one warm-up step" % (physics.upper(), n, k, el),
+                      "benchmark's %d^3 model, %d steps in %.1f s after one warm-up step" % (physics.upper(), n, k, el),
             "settings": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
                          "OMP_PLACES": os.environ.get("OMP_PLACES"), "ftz_daz": True,
                          "model_setup_s": round(setup_s, 1)},
@@ -775,7 +775,7 @@ def _cpu_child(physics, n, target_s):
 
 def cpu_baseline(physics, n, target_s=12.0):
     """Time the oracle's two-pass step (the reference's structure) on the host cores for about
-    target_s seconds on the configs' n^3 model, in a child process started with OMP_PROC_BIND=close
+    target_s seconds on the benchmark's n^3 model, in a child process started with OMP_PROC_BIND=close
     and OMP_PLACES=cores (benchmark/README.md:8) and flush-to-zero on; returns its JSON record."""
     env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
     cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", "--physics", physics, "--cpu-n", str(n),

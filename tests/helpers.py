@@ -112,7 +112,7 @@ def smooth_noise_big(shape, cells, seed):
 
 
 def config_model(physics, shape, d, dt, seed):
-    """The configs' model construction in oracle layout (nx, ny, nz): v(z) rising linearly from
+    """The benchmark model construction in oracle layout (nx, ny, nz): v(z) rising linearly from
     1500 m/s at the top to 4500 m/s at the bottom times 1 +- 5 % smooth noise, eps in [0, 0.3] and
     eta in [0, 0.15] (smooth), f = 0.85, velocity capped so that v sqrt(1 + 2 eps) dt / d <= 0.42,
     density from Gardner's relation clamped to [1, 2.5] g/cc, b = 1 / rho."""

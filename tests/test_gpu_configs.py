@@ -1,4 +1,4 @@
-"""GPU: the HIP path at the configs' own workloads (parity by size-independent properties).
+"""GPU: the HIP path at the benchmark workloads (parity by size-independent properties).
 
 * Iso 256^3 at 10 m, 2000 steps with a Hicks point source and 256 x 256 receivers at 20 m depth:
   traces and wavefields bit-identical to the oracle running the same loop on the host;
@@ -8,7 +8,7 @@
   src/prop3DAcoVTIDenQ_DEO2_FDTD.h:356-376, the step the reference's propagateadjoint! runs): the
   adjoint dot-product test <F s, d> = <s, F* d> <= 1e-5 for seeded Gaussian source and receiver
   vectors, with the weights dt^2 v^2 / b and its inverse at the points (the construction of
-  tests/test_gpu_adjoint.py), on the configs' seeded model;
+  tests/test_gpu_adjoint.py), on the benchmark's seeded model;
 * batched Born, Iso and VTI, on ginsu windows of a 1000 x 1000 x 600 model at 12.5 m (a 200^3
   window and a 600^3 one, nt = 40): two shots in one launch per step give the images and Born
   data of the two shots run one after the other, bit for bit; the Born dot test
@@ -19,7 +19,7 @@
   bytes, and to 5 levels, >= 11x; image relative L2 <= 1e-3 for both.
   nt = 400 here (the store of 3000 frames would not fit one GPU beside the exact reference run).
 
-The models follow the configs' construction (tests/helpers.py::config_model).
+The models follow the benchmark construction (tests/helpers.py::config_model).
 """
 import numpy as np
 import pytest
